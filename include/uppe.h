@@ -1,0 +1,154 @@
+/*
+ * C ABI of the MI355X ERK4(3)-IP UPPE propagators (libuppe.so, gfx950).
+ *
+ * Plain pointers and sizes only.  Device buffers (fields, workspace, outputs) are allocated by
+ * the caller; the library never allocates or frees device memory.  Every entry point returns
+ * 0 on success or a negative status; uppe_last_error() gives a thread-local message.  No C++
+ * exception crosses this boundary.  `stream` is a hipStream_t passed as void* (NULL = default
+ * stream).  Work is enqueued on that stream; only the *_status_get / *_run helpers synchronise.
+ *
+ * Reference interfaces each group replaces (paths relative to the reference root):
+ *   uppe3d_*  ERK43IP_UPPE_3D_Optimized.__init__ / propagate / erk43_step
+ *             src/3+1D-UPPE-ERK4(3)IP-envelope/solver_3d.py:8-54, 100-160, 162-233
+ *             and the medium's calc_nonlinear_response + plasma RawKernel
+ *             src/3+1D-UPPE-ERK4(3)IP-envelope/media_physics.py:11-67, 121-205
+ *   uppe1d_*  ERK43IP_UPPE.propagate / erk43_step / linear_operator / nonlinear_operator
+ *             src/1D-ERK4(3)IP UPPE gas/erk43ip_method.py:181-369 (argon patch gas.py:11-44)
+ */
+#ifndef UPPE_H
+#define UPPE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    UPPE_OK = 0,
+    UPPE_EINVAL = -1,   /* bad argument / unsupported size */
+    UPPE_EHIP = -2,     /* a HIP runtime call failed */
+    UPPE_ENONFINITE = -3 /* the error norm became NaN/Inf (the reference would loop forever) */
+};
+
+/* ------------------------------------------------------------------------------------------ */
+/* 3+1D envelope solver                                                                        */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct uppe3d_plan uppe3d_plan;
+
+typedef struct {
+    int nx, ny, nt;        /* grid; nx = ny in {16,32,64,128}, nt in {128,...,2048}, powers of 2 */
+    double dt;             /* t[1] - t[0] (s) */
+    double omega0;         /* carrier angular frequency (rad/s) */
+    double a_scale;        /* max|A0| (solver_3d.py:166-168); the state is A / a_scale */
+    /* medium scalars (media_physics.py:72-104 and subclasses) */
+    double n2, n4;         /* Kerr and HOKE coefficients (m^2/W, m^4/W^2) */
+    double Ui;             /* ionisation potential (J) */
+    double rho_nt;         /* neutral density (1/m^3) */
+    double tau_rec;        /* recombination time (s) */
+    double tau_c;          /* collision time (s) */
+    double sigma_drude;    /* Drude cross section (m^2) */
+    double av_coef;        /* avalanche rate = av_coef * I (0 for the gases) */
+    int use_shock;         /* shock operator (omega+omega0)/omega0 on the Kerr term */
+    /* ionisation LUT: W at lut_n points of log10(I) evenly spaced on [lut_x0, lut_x1] */
+    int lut_n;
+    double lut_x0, lut_x1;
+} uppe3d_desc;
+
+typedef struct {
+    double z, h, err;      /* position, next step size, last attempt's error norm */
+    int attempts, accepted, rejected, done, nonfinite, fail_count, cur;
+} uppe3d_status;
+
+/* Bytes of device workspace a plan needs (tables + 7 fields + control block). */
+int uppe3d_workspace_size(const uppe3d_desc *d, size_t *bytes);
+
+/* Builds a plan over caller-owned device workspace.  Per-omega arrays (length nt, FFT order,
+ * FP64, from the medium's refractive_index hook on the host): k(omega) = n*(omega0+omega)/c,
+ * k_ref = k0 + k1*omega, valid = (omega0+omega > 0), anti-alias filter, shock factor,
+ * valid-frequency mask.  kx2 (nx), ky2 (ny): squared transverse wavenumbers (FFT order).
+ * damp (nx*ny, x-major): absorber rate.  lut_w (lut_n): ionisation rate table. */
+int uppe3d_plan_create(const uppe3d_desc *d, void *workspace, size_t workspace_bytes,
+                       const double *k_w, const double *kref_w, const double *valid_w,
+                       const double *filt_w, const double *shock_w, const double *mask_w,
+                       const double *kx2, const double *ky2, const double *damp,
+                       const double *lut_w, void *stream, uppe3d_plan **out);
+int uppe3d_plan_destroy(uppe3d_plan *p);
+
+/* A0: complex64 (nx, ny, nt), t fastest, device.  Sets the state to FFT_t(A0 / a_scale). */
+int uppe3d_load_field(uppe3d_plan *p, const void *a0_xyt, void *stream);
+/* Starts an adaptive propagation over length L (solver_3d.py:172-180): z = 0,
+ * h = min(max_step, L/1000), controller constants.  Enqueued on `stream`. */
+int uppe3d_begin(uppe3d_plan *p, double L, double tol, double max_step, double min_step,
+                 void *stream);
+/* One ERK4(3)-IP attempt with the on-device controller (no host synchronisation). */
+int uppe3d_attempt(uppe3d_plan *p, void *stream);
+/* One attempt with a prescribed step h and accept decision (step-sequence replay). */
+int uppe3d_replay_attempt(uppe3d_plan *p, double h, int accept, void *stream);
+/* Runs attempts until done or max_attempts, reading the status every `check_every` attempts. */
+int uppe3d_run(uppe3d_plan *p, int max_attempts, int check_every, void *stream,
+               uppe3d_status *st);
+/* Synchronises `stream` and copies the control block. */
+int uppe3d_status_get(uppe3d_plan *p, void *stream, uppe3d_status *st);
+/* out: complex64 (nx, ny, nt) = fftshift_t(FFT_t(A)) * a_scale (solver_3d.py:215-233). */
+int uppe3d_store_spectrum(uppe3d_plan *p, void *out_xyw, void *stream);
+/* out: complex64 (nx, ny, nt) = A(x, y, t) * a_scale (time domain). */
+int uppe3d_store_field(uppe3d_plan *p, void *out_xyt, void *stream);
+/* Device pointer of the current state (complex64, layout [omega][x][y], divided by a_scale). */
+int uppe3d_state_ptr(uppe3d_plan *p, void *stream, void **state);
+
+/* ------------------------------------------------------------------------------------------ */
+/* 1D envelope solver (single instance and batched, one workgroup per instance)               */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct {
+    int nt;                /* power of two, 256..8192 */
+    double dt;
+    double omega0;
+    double gamma;          /* nonlinear coefficient (1/(W m)), read at every propagate */
+    int use_ss;            /* self-steepening with the UPPE factor (erk43ip_method.py:229-251) */
+    int use_raman;         /* I_eff = (1-f_R) I + f_R ifft(fft(I) H_R) (:206-217) */
+    double f_R;
+    int use_plasma;        /* plasma extension (1D restatement of media_physics.py:156-205) */
+    int K;                 /* multiphoton order: W = sigma_K * I^K, I = |A|^2 / area */
+    double sigma_K, Ui, rho_nt, tau_rec, tau_c, sigma_drude, area;
+    int avalanche;         /* W_ava = sigma_drude / Ui * I when set */
+    double L, tol, max_step, min_step;
+    int max_hist;          /* capacity of the history buffer (snapshots) */
+    int max_attempts;      /* safety cap on attempts (0 = unlimited) */
+} uppe1d_desc;
+
+typedef struct {
+    double z, h, err;
+    int attempts, accepted, rejected, forced, nhist, done, nonfinite, fsal_hits;
+} uppe1d_status;
+
+/* Device bytes needed by one instance for its tables and history. */
+int uppe1d_workspace_size(const uppe1d_desc *d, size_t *bytes);
+
+/* Propagates one pulse (erk43ip_method.py:288-369).  D (complex, nt), uppe factor (nt),
+ * H_R (complex, nt, or NULL) are FP64 host arrays in FFT order.  a0: complex64 (nt) device.
+ * hist: complex64 (max_hist, nt) device; zhist: float64 (max_hist) device.  Snapshot 0 is A0;
+ * the history rule of the reference appends after accepted steps.  st receives the result. */
+int uppe1d_propagate(const uppe1d_desc *d, const double *D_re, const double *D_im,
+                     const double *uppe_factor, const double *H_re, const double *H_im,
+                     const void *a0, void *workspace, void *hist, void *zhist, void *stream,
+                     uppe1d_status *st);
+
+/* Batched: n instances of uneven nt in one persistent launch, each with its own adaptive
+ * step.  descs[i], tables concatenated in order (offsets[i] = sum of nt before i), a0 and
+ * hist likewise concatenated (hist: max_hist[i]*nt[i] per instance), zhist max_hist[i]
+ * per instance.  threads: workgroup size chosen by the autotuner (256, 512 or 1024; 0 =
+ * library default).  Status for each instance in st[i]. */
+int uppe1d_propagate_batched(int n, const uppe1d_desc *descs, const double *D_re,
+                             const double *D_im, const double *uppe_factor,
+                             const double *H_re, const double *H_im, const void *a0,
+                             void *workspace, void *hist, void *zhist, int threads,
+                             void *stream, uppe1d_status *st);
+
+const char *uppe_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* UPPE_H */

@@ -1,0 +1,1156 @@
+// 3+1D envelope UPPE, ERK4(3) in the interaction picture, on gfx950.
+//
+// Replaces ERK43IP_UPPE_3D_Optimized.erk43_step / propagate
+// (src/3+1D-UPPE-ERK4(3)IP-envelope/solver_3d.py:100-233) and the medium's nonlinear
+// response with its plasma RawKernel (media_physics.py:11-67, 121-205).
+//
+// Layout.  Every field lives in HBM as complex64, plane-major: element (w, x, y) at
+// (w*nx + x)*ny + y, where w is the omega index in FFT order.  An xy plane is contiguous
+// (one workgroup holds it in LDS for a 2D FFT); a t line is strided by nx*ny and is read as
+// tiles of LINES consecutive y (LINES*8 bytes per omega row).
+//
+// Domains.  "M" = (x, y, omega) (t already transformed), "K" = (kx, ky, omega).  The state
+// between steps is B = A(x, y, omega) / a_scale in M; the reference's fft_t / ifft_t pair
+// around the step cancels, and the anti-alias filter (omega) and absorber (x, y) are both
+// diagonal in M.  One attempt with E = exp(h/2 D(kx, ky, omega)) and F = FFT2:
+//   T1: Y1 = N_M(B)                        (t-pass: iFFT_t, Kerr + plasma, FFT_t, mask)
+//   P2: V = E F(B); Q = E (F(B) + h/6 F(Y1)); X = F^-1(E (F(B) + h/2 F(Y1)))
+//   T2: Y2 = N_M(X)
+//   P3: X = F^-1(V + h/2 F(Y2)); Q += h/3 F(Y2)
+//   T3: Y3 = N_M(X)
+//   P4: X = F^-1(E (V + h F(Y3))); Q = E (Q + h/3 F(Y3))      (Q is now beta)
+//   T4: Y4 = N_M(X)
+//   P5: A4 = Q + h/6 F(Y4); max|A4|^2; X = F^-1(A4)
+//   T5: D = Y4 - N_M(X); B' = X * filter * exp(-damp h)
+//   P6: max|F(D)|^2;  err = (h/10) max|a4' - a5'| / (max|A4| + 1e-20)  = the reference's
+//       max|A4 - A3| / (max|A4| + 1e-20), since A4 - A3 = (h/10)(a4' - a5')
+//   controller (one thread, FP64): the accept / reject rule of solver_3d.py:182-208.
+// Field traffic per attempt: 31 complex64 field passes (2+5+2+5+2+5+2+3+4+1).
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <new>
+#include <string>
+#include <vector>
+
+#include "abi_common.hpp"
+#include "fft_common.hpp"
+
+using namespace uppe;
+
+namespace uppe {
+static thread_local std::string g_err;
+void set_error(const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+}  // namespace uppe
+
+extern "C" const char *uppe_last_error(void) { return uppe::g_err.c_str(); }
+
+namespace {
+
+struct Ctl3D {
+    double z, h, L, tol, max_step, min_step, err;
+    double replay_h;
+    int replay, replay_acc;
+    unsigned int maxA_bits, maxD_bits;
+    int cur, done, nonfinite, fail_count, attempts, accepted, rejected, pad_;
+};
+
+struct Tabs3D {
+    const float2 *tw_t, *tw_x, *tw_y;
+    const float *d_w, *k_w, *k2_w, *kref_w, *valid_w, *filt_w, *shock_w, *mask_w;
+    const float *kx2, *ky2, *damp;
+    const double *lut;
+};
+
+struct Phys3D {
+    float a_scale2;       // a_scale^2 (physical intensity = a_scale2 * |A|^2)
+    float g_n2, g_n4;     // (omega0/c) n2, (omega0/c) n4
+    float pad0;
+    double Ui, half_sigma, w0tc;
+    double rho_nt, r_rec, dt;   // float32-rounded values widened to double (launch casts)
+    double av_coef;
+    double lut_x0, lut_dx, lut_x1, lut_zero_below;
+    double e_rec, f_rec;        // exp(gdt) and step factor for Gamma = -r_rec
+    int lut_n, use_shock;
+};
+
+struct Fields3D {
+    float2 *B0, *B1, *X, *Yw, *Y4, *V, *Q;
+};
+
+__device__ __forceinline__ int pidx(int n) { return n + (n >> 4); }
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---------------------------------------------------------------------------------------
+// Per-line FFT held by one wavefront: N = 64*EPT points in LDS at pidx(n), natural order in
+// and out (Stockham autosort).  The line is private to the wave, so stages synchronise at
+// wavefront scope only.
+template <int N, int EPT, int NS, int DIR>
+struct LineFFT {
+    static constexpr int R = (N / NS >= EPT) ? EPT : (N / NS);
+    static constexpr int PER = EPT / R;
+    __device__ __forceinline__ static void run(float2 *line, int lane,
+                                               const float2 *__restrict__ tw) {
+        float2 v[PER][R];
+#pragma unroll
+        for (int p = 0; p < PER; ++p) {
+            const int j = lane + 64 * p;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[p][r] = line[pidx(j + r * (N / R))];
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int p = 0; p < PER; ++p) {
+            const int j = lane + 64 * p;
+            const int base = stockham_butterfly<R, DIR>(v[p], j, NS, N, tw);
+#pragma unroll
+            for (int r = 0; r < R; ++r) line[pidx(base + r * NS)] = v[p][r];
+        }
+        wave_lds_sync();
+        LineFFT<N, EPT, NS * R, DIR>::run(line, lane, tw);
+    }
+};
+template <int N, int EPT, int DIR>
+struct LineFFT<N, EPT, N, DIR> {
+    __device__ __forceinline__ static void run(float2 *, int, const float2 *) {}
+};
+
+// ---------------------------------------------------------------------------------------
+// Ionisation rate: np.interp on the log10(I) grid in FP64, then the float32 cast of
+// media_physics.py:131-137,169.
+__device__ __forceinline__ float lut_rate(float I, const double *__restrict__ lut,
+                                          const Phys3D &ph) {
+    double x = log10((double)I + 1e-30);
+    x = fmin(fmax(x, ph.lut_x0), ph.lut_x1);
+    int j = (int)((x - ph.lut_x0) / ph.lut_dx);
+    j = j < 0 ? 0 : (j > ph.lut_n - 2 ? ph.lut_n - 2 : j);
+    const double xj = ph.lut_x0 + j * ph.lut_dx;
+    const double xj1 = (j + 1 == ph.lut_n - 1) ? ph.lut_x1 : ph.lut_x0 + (j + 1) * ph.lut_dx;
+    const double f0 = lut[j], f1 = lut[j + 1];
+    return (float)((f1 - f0) / (xj1 - xj) * (x - xj) + f0);
+}
+
+// Plasma coefficients of one line held by a wavefront, lane owning t = lane*EPT + i.
+// Plasma term = -(c1 + i c2) A with c1 = gen Ui/(2I + 1e-30) + sigma/2 rho,
+// c2 = sigma/2 rho omega0 tau_c (media_physics.py:198-203).  rho/gen follow the recurrence of
+// media_physics.py:11-66 in FP64: an exact affine prefix scan across the wave when the
+// avalanche rate is zero and no clamp can act (w*f <= exp(gdt) for every sample); otherwise
+// lane 0 walks the line serially through the LDS line buffer.
+template <int NT, int EPT>
+__device__ void plasma_line(const float (&I)[EPT], const float (&wi)[EPT], float (&c1)[EPT],
+                            float (&c2)[EPT], float2 *line, int lane, const Phys3D &ph) {
+    const bool aval = ph.av_coef != 0.0;
+    bool risk = false;
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) risk |= ((double)wi[i] * ph.f_rec > ph.e_rec);
+    const bool serial = aval || (__ballot(risk) != 0);
+    if (!serial) {
+        double A = 1.0, Bc = 0.0;
+#pragma unroll
+        for (int i = 0; i < EPT; ++i) {
+            const double w = (double)wi[i];
+            const double ai = ph.e_rec - w * ph.f_rec;
+            Bc = ai * Bc + w * ph.rho_nt * ph.f_rec;
+            A = ai * A;
+        }
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const double Ao = __shfl_up(A, off, 64);
+            const double Bo = __shfl_up(Bc, off, 64);
+            if (lane >= off) {
+                Bc = A * Bo + Bc;
+                A = A * Ao;
+            }
+        }
+        double rho = __shfl_up(Bc, 1, 64);
+        if (lane == 0) rho = 0.0;
+#pragma unroll
+        for (int i = 0; i < EPT; ++i) {
+            const double w = (double)wi[i];
+            double depl = (ph.rho_nt - rho) / ph.rho_nt;
+            if (depl < 0.0) depl = 0.0;
+            const double gen = (w * ph.rho_nt) * depl;
+            const double cl = gen * ph.Ui / (2.0 * (double)I[i] + 1e-30);
+            const double cd = ph.half_sigma * rho;
+            c1[i] = (float)(cl + cd);
+            c2[i] = (float)(cd * ph.w0tc);
+            rho = rho * ph.e_rec + (w * ph.rho_nt * depl) * ph.f_rec;
+            rho = rho > ph.rho_nt ? ph.rho_nt : (rho < 0.0 ? 0.0 : rho);
+        }
+        return;
+    }
+    // serial path (reference order of operations, one lane)
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) line[pidx(lane * EPT + i)] = make_float2(wi[i], I[i]);
+    wave_lds_sync();
+    if (lane == 0) {
+        double rho = 0.0;
+        for (int t = 0; t < NT; ++t) {
+            const float2 p = line[pidx(t)];
+            const double w = (double)p.x;
+            const double wav = aval ? (double)(float)(ph.av_coef * (double)p.y) : 0.0;
+            double depl = (ph.rho_nt - rho) / ph.rho_nt;
+            if (depl < 0.0) depl = 0.0;
+            const double gen = (w * ph.rho_nt + wav * rho) * depl;
+            const double S = w * ph.rho_nt * depl;
+            const double G = wav * depl - ph.r_rec;
+            double gdt = G * ph.dt;
+            gdt = gdt > 10.0 ? 10.0 : (gdt < -10.0 ? -10.0 : gdt);
+            const double eg = exp(gdt);
+            const double sf = fabs(gdt) < 1e-5 ? ph.dt * (1.0 + 0.5 * gdt) : (eg - 1.0) / G;
+            const double cl = gen * ph.Ui / (2.0 * (double)p.y + 1e-30);
+            const double cd = ph.half_sigma * rho;
+            line[pidx(t)] = make_float2((float)(cl + cd), (float)(cd * ph.w0tc));
+            rho = rho * eg + S * sf;
+            rho = rho > ph.rho_nt ? ph.rho_nt : (rho < 0.0 ? 0.0 : rho);
+        }
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) {
+        const float2 p = line[pidx(lane * EPT + i)];
+        c1[i] = p.x;
+        c2[i] = p.y;
+    }
+    wave_lds_sync();
+}
+
+// ---------------------------------------------------------------------------------------
+// t-pass: Y = mask (shock FFT_t(Kerr) + FFT_t(Plasma)) of x = iFFT_t(X), LINES lines per
+// workgroup (fixed x, y0..y0+LINES-1), one wavefront per line.
+// MODE 0: out = Y.  MODE 1 (final stage): out = Y4 - Y, and during the load the next state
+// B' = X * filter(omega) * exp(-damp(x,y) h) is written to the spare state buffer.
+template <int NT, int LINES, int MODE>
+__global__ __launch_bounds__(64 * LINES) void tpass_kernel(Fields3D f, const float2 *in_fixed,
+                                                           int in_is_state, float2 *out,
+                                                           Tabs3D tb, Phys3D ph, Ctl3D *ctl,
+                                                           int nx, int ny) {
+    constexpr int EPT = NT / 64;
+    constexpr int LDT = NT + NT / 16;
+    constexpr int T = 64 * LINES;
+    __shared__ float2 lds[LINES * LDT];
+    if (ctl->done) return;
+    const int cur = ctl->cur;
+    const float2 *in = in_is_state ? (cur ? f.B1 : f.B0) : in_fixed;
+    const int tid = threadIdx.x;
+    const int bpx = ny / LINES;
+    const int x = blockIdx.x / bpx;
+    const int y0 = (blockIdx.x % bpx) * LINES;
+    const size_t plane = (size_t)nx * ny;
+    const size_t col0 = (size_t)x * ny + y0;
+    const int q = tid % LINES;
+    {
+        float absorb = 1.0f;
+        float2 *bnew = nullptr;
+        if (MODE == 1) {
+            absorb = expf(-tb.damp[col0 + q] * (float)ctl->h);
+            bnew = cur ? f.B0 : f.B1;
+        }
+#pragma unroll 8
+        for (int i = 0; i < NT * LINES / T; ++i) {
+            const int w = tid / LINES + (T / LINES) * i;
+            const size_t g = (size_t)w * plane + col0 + q;
+            const float2 v = in[g];
+            lds[q * LDT + pidx(w)] = v;
+            if (MODE == 1) bnew[g] = cscale(v, tb.filt_w[w] * absorb);
+        }
+    }
+    __syncthreads();
+    const int wv = tid >> 6, lane = tid & 63;
+    float2 *line = lds + wv * LDT;
+    LineFFT<NT, EPT, 1, -1>::run(line, lane, tb.tw_t);
+    float2 a[EPT];
+    float I[EPT], wi[EPT], c1[EPT], c2[EPT];
+    const float invN = 1.0f / NT;
+    bool any = false;
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) {
+        a[i] = cscale(line[pidx(lane * EPT + i)], invN);
+        I[i] = ph.a_scale2 * cabs2(a[i]);
+        float w = 0.0f;
+        if ((double)I[i] > ph.lut_zero_below) w = lut_rate(I[i], tb.lut, ph);
+        wi[i] = w;
+        any |= (w > 0.0f);
+        c1[i] = 0.0f;
+        c2[i] = 0.0f;
+    }
+    const bool plasma = __ballot(any) != 0;
+    if (plasma) plasma_line<NT, EPT>(I, wi, c1, c2, line, lane, ph);
+    if (ph.use_shock) {
+        // Kerr alone, shock applied in omega; plasma (if any) in a second transform
+#pragma unroll
+        for (int i = 0; i < EPT; ++i) {
+            const float kap = I[i] * (ph.g_n2 + ph.g_n4 * I[i]);
+            line[pidx(lane * EPT + i)] = make_float2(-kap * a[i].y, kap * a[i].x);
+        }
+        wave_lds_sync();
+        LineFFT<NT, EPT, 1, 1>::run(line, lane, tb.tw_t);
+        float2 acc[EPT];
+#pragma unroll
+        for (int i = 0; i < EPT; ++i) {
+            const int w = lane + 64 * i;
+            acc[i] = cscale(line[pidx(w)], tb.shock_w[w] * tb.mask_w[w]);
+        }
+        if (plasma) {
+            wave_lds_sync();
+#pragma unroll
+            for (int i = 0; i < EPT; ++i)
+                line[pidx(lane * EPT + i)] =
+                    make_float2(-c1[i] * a[i].x + c2[i] * a[i].y, -c1[i] * a[i].y - c2[i] * a[i].x);
+            wave_lds_sync();
+            LineFFT<NT, EPT, 1, 1>::run(line, lane, tb.tw_t);
+#pragma unroll
+            for (int i = 0; i < EPT; ++i) {
+                const int w = lane + 64 * i;
+                acc[i] = caxpy(acc[i], tb.mask_w[w], line[pidx(w)]);
+            }
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int i = 0; i < EPT; ++i) line[pidx(lane + 64 * i)] = acc[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < EPT; ++i) {
+            const float kap = I[i] * (ph.g_n2 + ph.g_n4 * I[i]);
+            const float im = kap - c2[i];
+            line[pidx(lane * EPT + i)] =
+                make_float2(-c1[i] * a[i].x - im * a[i].y, -c1[i] * a[i].y + im * a[i].x);
+        }
+        wave_lds_sync();
+        LineFFT<NT, EPT, 1, 1>::run(line, lane, tb.tw_t);
+#pragma unroll
+        for (int i = 0; i < EPT; ++i) {
+            const int w = lane + 64 * i;
+            line[pidx(w)] = cscale(line[pidx(w)], tb.mask_w[w]);
+        }
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int i = 0; i < NT * LINES / T; ++i) {
+        const int w = tid / LINES + (T / LINES) * i;
+        const size_t g = (size_t)w * plane + col0 + q;
+        float2 v = lds[q * LDT + pidx(w)];
+        if (MODE == 1) v = csub(f.Y4[g], v);
+        out[g] = v;
+    }
+}
+
+// (x, y, t) input -> state B[cur] = FFT_t(A0) / a_scale in the plane-major layout.
+template <int NT, int LINES>
+__global__ __launch_bounds__(64 * LINES) void load_kernel(const float2 *__restrict__ a0,
+                                                          Fields3D f, Tabs3D tb, Ctl3D *ctl,
+                                                          float inv_scale, int nx, int ny) {
+    constexpr int EPT = NT / 64;
+    constexpr int LDT = NT + NT / 16;
+    constexpr int T = 64 * LINES;
+    __shared__ float2 lds[LINES * LDT];
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int bpx = ny / LINES;
+    const int x = blockIdx.x / bpx, y0 = (blockIdx.x % bpx) * LINES;
+    float2 *line = lds + wv * LDT;
+    const float2 *src = a0 + ((size_t)x * ny + y0 + wv) * NT;
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) line[pidx(lane + 64 * i)] = cscale(src[lane + 64 * i], inv_scale);
+    wave_lds_sync();
+    LineFFT<NT, EPT, 1, 1>::run(line, lane, tb.tw_t);
+    __syncthreads();
+    float2 *B = ctl->cur ? f.B1 : f.B0;
+    const size_t plane = (size_t)nx * ny, col0 = (size_t)x * ny + y0;
+    const int q = tid % LINES;
+    for (int i = 0; i < NT * LINES / T; ++i) {
+        const int w = tid / LINES + (T / LINES) * i;
+        B[(size_t)w * plane + col0 + q] = lds[q * LDT + pidx(w)];
+    }
+}
+
+// state -> (x, y, t) output.  SPECTRUM: fftshift_t(B) * scale; else iFFT_t(B) * scale.
+template <int NT, int LINES, bool SPECTRUM>
+__global__ __launch_bounds__(64 * LINES) void store_kernel(float2 *__restrict__ out, Fields3D f,
+                                                           Tabs3D tb, Ctl3D *ctl, float scale,
+                                                           int nx, int ny) {
+    constexpr int EPT = NT / 64;
+    constexpr int LDT = NT + NT / 16;
+    constexpr int T = 64 * LINES;
+    __shared__ float2 lds[LINES * LDT];
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int bpx = ny / LINES;
+    const int x = blockIdx.x / bpx, y0 = (blockIdx.x % bpx) * LINES;
+    const float2 *B = ctl->cur ? f.B1 : f.B0;
+    const size_t plane = (size_t)nx * ny, col0 = (size_t)x * ny + y0;
+    const int q = tid % LINES;
+    for (int i = 0; i < NT * LINES / T; ++i) {
+        const int w = tid / LINES + (T / LINES) * i;
+        lds[q * LDT + pidx(w)] = B[(size_t)w * plane + col0 + q];
+    }
+    __syncthreads();
+    float2 *line = lds + wv * LDT;
+    float2 *dst = out + ((size_t)x * ny + y0 + wv) * NT;
+    if (SPECTRUM) {
+#pragma unroll
+        for (int i = 0; i < EPT; ++i) {
+            const int w = lane + 64 * i;
+            dst[(w + NT / 2) & (NT - 1)] = cscale(line[pidx(w)], scale);
+        }
+    } else {
+        LineFFT<NT, EPT, 1, -1>::run(line, lane, tb.tw_t);
+#pragma unroll
+        for (int i = 0; i < EPT; ++i) {
+            const int t = lane + 64 * i;
+            dst[t] = cscale(line[pidx(t)], scale / NT);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// xy planes: one workgroup per omega plane, the plane in LDS with rows padded to LD.
+constexpr int xy_threads(int nx, int ny) {
+    return nx * ny >= 16384 ? 1024 : (nx * ny / 16 >= 64 ? nx * ny / 16 : 64);
+}
+
+template <int NX, int NY>
+struct XY {
+    static constexpr int P = NX * NY;
+    static constexpr int T = P >= 16384 ? 1024 : (P / 16 >= 64 ? P / 16 : 64);
+    static constexpr int EPT = P / T;
+    static constexpr int LD = NY + 1;
+};
+
+// FFTs along y (contiguous within a row): NX transforms of length NY.
+template <int NX, int NY, int T, int NS, int DIR>
+struct RowFFT {
+    static constexpr int LD = NY + 1;
+    static constexpr int R = (NY / NS >= 16) ? 16 : NY / NS;
+    static constexpr int NB = NX * (NY / R);
+    static constexpr int PER = (NB + T - 1) / T;
+    __device__ __forceinline__ static void run(float2 *lds, int tid, const float2 *__restrict__ tw) {
+        float2 v[PER][R];
+#pragma unroll
+        for (int p = 0; p < PER; ++p) {
+            const int u = tid + T * p;
+            if (u < NB) {
+                const int j = u % (NY / R), x = u / (NY / R);
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[p][r] = lds[x * LD + j + r * (NY / R)];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < PER; ++p) {
+            const int u = tid + T * p;
+            if (u < NB) {
+                const int j = u % (NY / R), x = u / (NY / R);
+                const int b = stockham_butterfly<R, DIR>(v[p], j, NS, NY, tw);
+#pragma unroll
+                for (int r = 0; r < R; ++r) lds[x * LD + b + r * NS] = v[p][r];
+            }
+        }
+        __syncthreads();
+        RowFFT<NX, NY, T, NS * R, DIR>::run(lds, tid, tw);
+    }
+};
+template <int NX, int NY, int T, int DIR>
+struct RowFFT<NX, NY, T, NY, DIR> {
+    __device__ __forceinline__ static void run(float2 *, int, const float2 *) {}
+};
+
+// FFTs along x (stride LD): NY transforms of length NX; consecutive threads take
+// consecutive y, so LDS accesses are contiguous.
+template <int NX, int NY, int T, int NS, int DIR>
+struct ColFFT {
+    static constexpr int LD = NY + 1;
+    static constexpr int R = (NX / NS >= 16) ? 16 : NX / NS;
+    static constexpr int NB = NY * (NX / R);
+    static constexpr int PER = (NB + T - 1) / T;
+    __device__ __forceinline__ static void run(float2 *lds, int tid, const float2 *__restrict__ tw) {
+        float2 v[PER][R];
+#pragma unroll
+        for (int p = 0; p < PER; ++p) {
+            const int u = tid + T * p;
+            if (u < NB) {
+                const int y = u % NY, j = u / NY;
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[p][r] = lds[(j + r * (NX / R)) * LD + y];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < PER; ++p) {
+            const int u = tid + T * p;
+            if (u < NB) {
+                const int y = u % NY, j = u / NY;
+                const int b = stockham_butterfly<R, DIR>(v[p], j, NS, NX, tw);
+#pragma unroll
+                for (int r = 0; r < R; ++r) lds[(b + r * NS) * LD + y] = v[p][r];
+            }
+        }
+        __syncthreads();
+        ColFFT<NX, NY, T, NS * R, DIR>::run(lds, tid, tw);
+    }
+};
+template <int NX, int NY, int T, int DIR>
+struct ColFFT<NX, NY, T, NX, DIR> {
+    __device__ __forceinline__ static void run(float2 *, int, const float2 *) {}
+};
+
+// 2D FFT of a plane held in registers (element e = tid + T*i, natural x-major order).
+// Forward unnormalised; inverse scaled by 1/(NX*NY).
+template <int NX, int NY, int DIR, int E>
+__device__ __forceinline__ void fft2(float2 (&a)[E], float2 *lds, int tid,
+                                     const Tabs3D &tb) {
+    using C = XY<NX, NY>;
+#pragma unroll
+    for (int i = 0; i < C::EPT; ++i) {
+        const int e = tid + C::T * i;
+        lds[(e / NY) * C::LD + (e % NY)] = a[i];
+    }
+    __syncthreads();
+    RowFFT<NX, NY, C::T, 1, DIR>::run(lds, tid, tb.tw_y);
+    ColFFT<NX, NY, C::T, 1, DIR>::run(lds, tid, tb.tw_x);
+    const float s = DIR > 0 ? 1.0f : 1.0f / (float)(NX * NY);
+#pragma unroll
+    for (int i = 0; i < C::EPT; ++i) {
+        const int e = tid + C::T * i;
+        a[i] = cscale(lds[(e / NY) * C::LD + (e % NY)], s);
+    }
+    __syncthreads();
+}
+
+// exp(s * D(kx, ky, omega_w)) with D = i (kz - k_ref), kz the principal sqrt of k^2 - kperp^2,
+// written as D = i (d_w - kperp^2 / (kz + k)) for kz real (d_w = k - k_ref) and
+// D = -sqrt(kperp^2 - k^2) - i k_ref when evanescent; D = 0 where omega0 + omega <= 0
+// (solver_3d.py:56-87).
+template <int NY>
+__device__ __forceinline__ float2 eprop(int e, int w, float s, const Tabs3D &tb) {
+    if (tb.valid_w[w] == 0.0f) return make_float2(1.0f, 0.0f);
+    const float kp2 = tb.kx2[e / NY] + tb.ky2[e % NY];
+    const float k2 = tb.k2_w[w];
+    float dre = 0.0f, dim;
+    if (kp2 <= k2) {
+        const float kz = sqrtf(k2 - kp2);
+        dim = tb.d_w[w] - kp2 / (kz + tb.k_w[w]);
+    } else {
+        dre = -sqrtf(kp2 - k2);
+        dim = -tb.kref_w[w];
+    }
+    float sn, cs;
+    sincosf(s * dim, &sn, &cs);
+    const float mag = dre == 0.0f ? 1.0f : expf(s * dre);
+    return make_float2(mag * cs, mag * sn);
+}
+
+template <int T>
+__device__ __forceinline__ void block_max_to(unsigned int m, unsigned int *dst, unsigned int *red,
+                                             int tid) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned int v = __shfl_xor(m, o, 64);
+        m = v > m ? v : m;
+    }
+    if ((tid & 63) == 0) red[tid >> 6] = m;
+    __syncthreads();
+    if (tid == 0) {
+        unsigned int r = red[0];
+        for (int i = 1; i < T / 64; ++i) r = red[i] > r ? red[i] : r;
+        atomicMax(dst, r);
+    }
+}
+
+template <int NX, int NY, int MODE>
+__global__ __launch_bounds__(xy_threads(NX, NY)) void xypass_kernel(Fields3D f, Tabs3D tb,
+                                                               Ctl3D *ctl) {
+    using C = XY<NX, NY>;
+    __shared__ float2 lds[NX * C::LD];
+    __shared__ unsigned int red[C::T / 64 > 0 ? C::T / 64 : 1];
+    if (ctl->done) return;
+    const int w = blockIdx.x;
+    const int tid = threadIdx.x;
+    const size_t pb = (size_t)w * C::P;
+    const float h = (float)ctl->h;
+    float2 a[C::EPT], b[C::EPT];
+    if (MODE == 2) {
+        const float2 *B = ctl->cur ? f.B1 : f.B0;
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) {
+            a[i] = B[pb + tid + C::T * i];
+            b[i] = f.Yw[pb + tid + C::T * i];
+        }
+        fft2<NX, NY, 1>(a, lds, tid, tb);
+        fft2<NX, NY, 1>(b, lds, tid, tb);
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) {
+            const int e = tid + C::T * i;
+            const float2 E = eprop<NY>(e, w, 0.5f * h, tb);
+            f.V[pb + e] = cmul(E, a[i]);
+            f.Q[pb + e] = cmul(E, caxpy(a[i], h / 6.0f, b[i]));
+            a[i] = cmul(E, caxpy(a[i], 0.5f * h, b[i]));
+        }
+        fft2<NX, NY, -1>(a, lds, tid, tb);
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) f.X[pb + tid + C::T * i] = a[i];
+    } else if (MODE == 3 || MODE == 4) {
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) b[i] = f.Yw[pb + tid + C::T * i];
+        fft2<NX, NY, 1>(b, lds, tid, tb);
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) {
+            const int e = tid + C::T * i;
+            const float2 v = f.V[pb + e];
+            const float2 qv = f.Q[pb + e];
+            if (MODE == 3) {
+                a[i] = caxpy(v, 0.5f * h, b[i]);
+                f.Q[pb + e] = caxpy(qv, h / 3.0f, b[i]);
+            } else {
+                const float2 E = eprop<NY>(e, w, 0.5f * h, tb);
+                a[i] = cmul(E, caxpy(v, h, b[i]));
+                f.Q[pb + e] = cmul(E, caxpy(qv, h / 3.0f, b[i]));
+            }
+        }
+        fft2<NX, NY, -1>(a, lds, tid, tb);
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) f.X[pb + tid + C::T * i] = a[i];
+    } else if (MODE == 5) {
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) b[i] = f.Y4[pb + tid + C::T * i];
+        fft2<NX, NY, 1>(b, lds, tid, tb);
+        unsigned int m = 0;
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) {
+            const int e = tid + C::T * i;
+            a[i] = caxpy(f.Q[pb + e], h / 6.0f, b[i]);
+            const unsigned int v = __float_as_uint(cabs2(a[i]));
+            m = v > m ? v : m;
+        }
+        block_max_to<C::T>(m, &ctl->maxA_bits, red, tid);
+        fft2<NX, NY, -1>(a, lds, tid, tb);
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) f.X[pb + tid + C::T * i] = a[i];
+    } else {  // MODE 6
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) b[i] = f.Yw[pb + tid + C::T * i];
+        fft2<NX, NY, 1>(b, lds, tid, tb);
+        unsigned int m = 0;
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) {
+            const unsigned int v = __float_as_uint(cabs2(b[i]));
+            m = v > m ? v : m;
+        }
+        block_max_to<C::T>(m, &ctl->maxD_bits, red, tid);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Step controller (solver_3d.py:182-208), one thread, FP64.  Also applies the top-of-loop
+// clip h = min(h, L - z) of the next iteration and the h < 1e-15 stop.
+__global__ void ctl_kernel(Ctl3D *c) {
+    if (c->done) return;
+    const double h = c->h;
+    const unsigned int ab = c->maxA_bits, db = c->maxD_bits;
+    c->maxA_bits = 0;
+    c->maxD_bits = 0;
+    c->attempts += 1;
+    // An overflowed norm (FP32 range; the FP64 reference would see a huge finite error)
+    // counts as err = +inf: the attempt is rejected and h falls to min_step, the value the
+    // reference's factor (tol/err)^0.2 gives for any err that large.  Accepting such an
+    // attempt (forced min-step or replayed accept) stops the run as non-finite.
+    const bool bad = ab >= 0x7f800000u || db >= 0x7f800000u;
+    const double err = bad ? __longlong_as_double(0x7ff0000000000000ll)
+                           : (h / 10.0) * sqrt((double)__uint_as_float(db)) /
+                                 (sqrt((double)__uint_as_float(ab)) + 1e-20);
+    c->err = err;
+    const bool forced = h <= c->min_step * 1.01;
+    const bool acc = c->replay ? (c->replay_acc != 0) : (err < c->tol || forced);
+    if (bad && acc) {
+        c->nonfinite = 1;
+        c->done = 1;
+        return;
+    }
+    double hn;
+    if (acc) {
+        c->fail_count = (forced && err > c->tol) ? c->fail_count + 1 : 0;
+        c->z += h;
+        c->cur ^= 1;
+        c->accepted += 1;
+        const double e = err > 1e-30 ? err : 1e-30;
+        double fac = 0.95 * pow(c->tol / e, 0.2);
+        if (c->fail_count > 0) fac = fac < 1.2 ? fac : 1.2;
+        hn = h * (fac < 2.0 ? fac : 2.0);
+        hn = hn < c->max_step ? hn : c->max_step;
+    } else {
+        c->rejected += 1;
+        const double e = err > 1e-30 ? err : 1e-30;
+        const double fac = 0.95 * pow(c->tol / e, 0.2);
+        hn = h * fac;
+        hn = hn > c->min_step ? hn : c->min_step;
+    }
+    if (c->replay) hn = c->replay_h;
+    if (!(c->z < c->L)) {
+        c->done = 1;
+    } else if (hn > c->L - c->z) {
+        hn = c->L - c->z;
+        if (hn < 1e-15) c->done = 1;
+    }
+    c->h = hn;
+}
+
+__global__ void begin_kernel(Ctl3D *c, double L, double tol, double max_step, double min_step) {
+    c->z = 0.0;
+    c->err = 0.0;
+    c->replay_h = 0.0;
+    c->replay = 0;
+    c->replay_acc = 0;
+    c->maxA_bits = 0;
+    c->maxD_bits = 0;
+    c->done = 0;
+    c->nonfinite = 0;
+    c->fail_count = 0;
+    c->attempts = 0;
+    c->accepted = 0;
+    c->rejected = 0;
+    c->L = L;
+    c->tol = tol;
+    c->max_step = max_step;
+    c->min_step = min_step;
+    double h = max_step < L / 1000.0 ? max_step : L / 1000.0;
+    if (!(0.0 < L)) c->done = 1;
+    else if (h > L) h = L;
+    c->h = h;
+}
+
+__global__ void replay_set_kernel(Ctl3D *c, double h, int acc, double h_next) {
+    c->replay = 1;
+    c->replay_acc = acc;
+    c->h = h;
+    c->replay_h = h_next;
+    c->done = 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// Host side
+struct uppe3d_plan {
+    uppe3d_desc d;
+    Fields3D f;
+    Tabs3D tb;
+    Phys3D ph;
+    Ctl3D *ctl;
+    size_t ws_bytes;
+};
+
+namespace {
+
+constexpr size_t kAlign = 256;
+size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+struct Layout3D {
+    size_t tw_t, tw_x, tw_y, d_w, k_w, k2_w, kref_w, valid_w, filt_w, shock_w, mask_w, kx2, ky2,
+        damp, lut, ctl, fields[7], total;
+};
+
+Layout3D layout3d(const uppe3d_desc &d) {
+    Layout3D L;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        size_t at = o;
+        o = align_up(o + bytes);
+        return at;
+    };
+    L.tw_t = take(sizeof(float2) * d.nt);
+    L.tw_x = take(sizeof(float2) * d.nx);
+    L.tw_y = take(sizeof(float2) * d.ny);
+    L.d_w = take(4 * d.nt);
+    L.k_w = take(4 * d.nt);
+    L.k2_w = take(4 * d.nt);
+    L.kref_w = take(4 * d.nt);
+    L.valid_w = take(4 * d.nt);
+    L.filt_w = take(4 * d.nt);
+    L.shock_w = take(4 * d.nt);
+    L.mask_w = take(4 * d.nt);
+    L.kx2 = take(4 * d.nx);
+    L.ky2 = take(4 * d.ny);
+    L.damp = take(4 * (size_t)d.nx * d.ny);
+    L.lut = take(8 * (size_t)d.lut_n);
+    L.ctl = take(sizeof(Ctl3D));
+    const size_t field = sizeof(float2) * (size_t)d.nx * d.ny * d.nt;
+    for (int i = 0; i < 7; ++i) L.fields[i] = take(field);
+    L.total = o;
+    return L;
+}
+
+bool pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+int check_desc(const uppe3d_desc *d) {
+    UPPE_CHECK(d != nullptr, "uppe3d: null descriptor");
+    UPPE_CHECK(d->nx == d->ny, "uppe3d: nx (%d) must equal ny (%d)", d->nx, d->ny);
+    UPPE_CHECK(d->nx == 16 || d->nx == 32 || d->nx == 64 || d->nx == 128,
+               "uppe3d: nx = %d unsupported (16, 32, 64, 128)", d->nx);
+    UPPE_CHECK(pow2(d->nt) && d->nt >= 128 && d->nt <= 2048,
+               "uppe3d: nt = %d unsupported (power of two in [128, 2048])", d->nt);
+    UPPE_CHECK(d->lut_n >= 2, "uppe3d: lut_n must be >= 2");
+    UPPE_CHECK(d->dt > 0.0 && d->a_scale > 0.0, "uppe3d: dt and a_scale must be positive");
+    return UPPE_OK;
+}
+
+void twiddles(std::vector<float2> &v, int n) {
+    v.resize(n);
+    for (int k = 0; k < n; ++k) {
+        const double a = -2.0 * M_PI * (double)k / (double)n;
+        v[k] = make_float2((float)cos(a), (float)sin(a));
+    }
+}
+
+#define LAUNCH_T(NT, LINES)                                                                    \
+    case NT:                                                                                   \
+        if (mode == 0)                                                                         \
+            tpass_kernel<NT, LINES, 0><<<grid, 64 * LINES, 0, s>>>(p->f, in, in_is_state, out, \
+                                                                  p->tb, p->ph, p->ctl, nx,    \
+                                                                  ny);                         \
+        else                                                                                   \
+            tpass_kernel<NT, LINES, 1><<<grid, 64 * LINES, 0, s>>>(p->f, in, in_is_state, out, \
+                                                                  p->tb, p->ph, p->ctl, nx,    \
+                                                                  ny);                         \
+        break;
+
+int launch_tpass(uppe3d_plan *p, int mode, const float2 *in, int in_is_state, float2 *out,
+                 hipStream_t s) {
+    const int nx = p->d.nx, ny = p->d.ny;
+    const int lines = p->d.nt >= 2048 ? 8 : 16;
+    const dim3 grid(nx * (ny / lines));
+    switch (p->d.nt) {
+        LAUNCH_T(128, 16)
+        LAUNCH_T(256, 16)
+        LAUNCH_T(512, 16)
+        LAUNCH_T(1024, 16)
+        LAUNCH_T(2048, 8)
+        default:
+            return UPPE_EINVAL;
+    }
+    UPPE_HIP(hipGetLastError());
+    return UPPE_OK;
+}
+
+template <int N>
+int launch_xy_n(uppe3d_plan *p, int mode, hipStream_t s) {
+    const dim3 grid(p->d.nt), block(XY<N, N>::T);
+    switch (mode) {
+        case 2: xypass_kernel<N, N, 2><<<grid, block, 0, s>>>(p->f, p->tb, p->ctl); break;
+        case 3: xypass_kernel<N, N, 3><<<grid, block, 0, s>>>(p->f, p->tb, p->ctl); break;
+        case 4: xypass_kernel<N, N, 4><<<grid, block, 0, s>>>(p->f, p->tb, p->ctl); break;
+        case 5: xypass_kernel<N, N, 5><<<grid, block, 0, s>>>(p->f, p->tb, p->ctl); break;
+        default: xypass_kernel<N, N, 6><<<grid, block, 0, s>>>(p->f, p->tb, p->ctl); break;
+    }
+    UPPE_HIP(hipGetLastError());
+    return UPPE_OK;
+}
+
+int launch_xy(uppe3d_plan *p, int mode, hipStream_t s) {
+    switch (p->d.nx) {
+        case 16: return launch_xy_n<16>(p, mode, s);
+        case 32: return launch_xy_n<32>(p, mode, s);
+        case 64: return launch_xy_n<64>(p, mode, s);
+        case 128: return launch_xy_n<128>(p, mode, s);
+    }
+    return UPPE_EINVAL;
+}
+
+#define RET(x)                       \
+    do {                             \
+        int r_ = (x);                \
+        if (r_ != UPPE_OK) return r_; \
+    } while (0)
+
+int enqueue_attempt(uppe3d_plan *p, hipStream_t s) {
+    Fields3D &f = p->f;
+    RET(launch_tpass(p, 0, nullptr, 1, f.Yw, s));   // T1 on the state
+    RET(launch_xy(p, 2, s));                          // P2
+    RET(launch_tpass(p, 0, f.X, 0, f.Yw, s));        // T2
+    RET(launch_xy(p, 3, s));                          // P3
+    RET(launch_tpass(p, 0, f.X, 0, f.Yw, s));        // T3
+    RET(launch_xy(p, 4, s));                          // P4
+    RET(launch_tpass(p, 0, f.X, 0, f.Y4, s));        // T4
+    RET(launch_xy(p, 5, s));                          // P5
+    RET(launch_tpass(p, 1, f.X, 0, f.Yw, s));        // T5 (D and next state)
+    RET(launch_xy(p, 6, s));                          // P6
+    ctl_kernel<<<1, 1, 0, s>>>(p->ctl);
+    UPPE_HIP(hipGetLastError());
+    return UPPE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int uppe3d_workspace_size(const uppe3d_desc *d, size_t *bytes) {
+    RET(check_desc(d));
+    UPPE_CHECK(bytes != nullptr, "uppe3d_workspace_size: null output");
+    *bytes = layout3d(*d).total;
+    return UPPE_OK;
+}
+
+int uppe3d_plan_create(const uppe3d_desc *d, void *workspace, size_t workspace_bytes,
+                       const double *k_w, const double *kref_w, const double *valid_w,
+                       const double *filt_w, const double *shock_w, const double *mask_w,
+                       const double *kx2, const double *ky2, const double *damp,
+                       const double *lut_w, void *stream, uppe3d_plan **out) {
+    RET(check_desc(d));
+    UPPE_CHECK(out && workspace && k_w && kref_w && valid_w && filt_w && shock_w && mask_w &&
+                   kx2 && ky2 && damp && lut_w,
+               "uppe3d_plan_create: null argument");
+    const Layout3D L = layout3d(*d);
+    UPPE_CHECK(workspace_bytes >= L.total, "uppe3d_plan_create: workspace %zu < %zu bytes",
+               workspace_bytes, L.total);
+    UPPE_CHECK(((uintptr_t)workspace % kAlign) == 0, "uppe3d_plan_create: workspace not 256-B aligned");
+    hipStream_t s = (hipStream_t)stream;
+    char *ws = (char *)workspace;
+    const int nt = d->nt, nx = d->nx, ny = d->ny;
+    std::vector<float2> twt, twx, twy;
+    twiddles(twt, nt);
+    twiddles(twx, nx);
+    twiddles(twy, ny);
+    std::vector<float> dw(nt), kw(nt), k2w(nt), krw(nt), vw(nt), fw(nt), sw(nt), mw(nt);
+    for (int i = 0; i < nt; ++i) {
+        dw[i] = (float)(k_w[i] - kref_w[i]);
+        kw[i] = (float)k_w[i];
+        k2w[i] = (float)(k_w[i] * k_w[i]);
+        krw[i] = (float)kref_w[i];
+        vw[i] = (float)valid_w[i];
+        fw[i] = (float)filt_w[i];
+        sw[i] = (float)shock_w[i];
+        mw[i] = (float)mask_w[i];
+    }
+    std::vector<float> kx(nx), ky(ny), dm((size_t)nx * ny);
+    for (int i = 0; i < nx; ++i) kx[i] = (float)kx2[i];
+    for (int i = 0; i < ny; ++i) ky[i] = (float)ky2[i];
+    for (size_t i = 0; i < (size_t)nx * ny; ++i) dm[i] = (float)damp[i];
+    // copies are synchronous so the host vectors may go out of scope
+    UPPE_HIP(hipMemcpy(ws + L.tw_t, twt.data(), sizeof(float2) * nt, hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(ws + L.tw_x, twx.data(), sizeof(float2) * nx, hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(ws + L.tw_y, twy.data(), sizeof(float2) * ny, hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(ws + L.d_w, dw.data(), 4 * nt, hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(ws + L.k_w, kw.data(), 4 * nt, hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(ws + L.k2_w, k2w.data(), 4 * nt, hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(ws + L.kref_w, krw.data(), 4 * nt, hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(ws + L.valid_w, vw.data(), 4 * nt, hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(ws + L.filt_w, fw.data(), 4 * nt, hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(ws + L.shock_w, sw.data(), 4 * nt, hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(ws + L.mask_w, mw.data(), 4 * nt, hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(ws + L.kx2, kx.data(), 4 * nx, hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(ws + L.ky2, ky.data(), 4 * ny, hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(ws + L.damp, dm.data(), 4 * (size_t)nx * ny, hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(ws + L.lut, lut_w, 8 * (size_t)d->lut_n, hipMemcpyHostToDevice));
+    Ctl3D c0;
+    memset(&c0, 0, sizeof c0);
+    c0.done = 1;
+    UPPE_HIP(hipMemcpy(ws + L.ctl, &c0, sizeof c0, hipMemcpyHostToDevice));
+
+    uppe3d_plan *p = new (std::nothrow) uppe3d_plan;
+    UPPE_CHECK(p != nullptr, "uppe3d_plan_create: out of host memory");
+    p->d = *d;
+    p->ws_bytes = L.total;
+    p->ctl = (Ctl3D *)(ws + L.ctl);
+    float2 **fp[7] = {&p->f.B0, &p->f.B1, &p->f.X, &p->f.Yw, &p->f.Y4, &p->f.V, &p->f.Q};
+    for (int i = 0; i < 7; ++i) *fp[i] = (float2 *)(ws + L.fields[i]);
+    Tabs3D &tb = p->tb;
+    tb.tw_t = (const float2 *)(ws + L.tw_t);
+    tb.tw_x = (const float2 *)(ws + L.tw_x);
+    tb.tw_y = (const float2 *)(ws + L.tw_y);
+    tb.d_w = (const float *)(ws + L.d_w);
+    tb.k_w = (const float *)(ws + L.k_w);
+    tb.k2_w = (const float *)(ws + L.k2_w);
+    tb.kref_w = (const float *)(ws + L.kref_w);
+    tb.valid_w = (const float *)(ws + L.valid_w);
+    tb.filt_w = (const float *)(ws + L.filt_w);
+    tb.shock_w = (const float *)(ws + L.shock_w);
+    tb.mask_w = (const float *)(ws + L.mask_w);
+    tb.kx2 = (const float *)(ws + L.kx2);
+    tb.ky2 = (const float *)(ws + L.ky2);
+    tb.damp = (const float *)(ws + L.damp);
+    tb.lut = (const double *)(ws + L.lut);
+
+    Phys3D &ph = p->ph;
+    memset(&ph, 0, sizeof ph);
+    const double c = 299792458.0;
+    ph.a_scale2 = (float)(d->a_scale * d->a_scale);
+    ph.g_n2 = (float)(d->omega0 / c * d->n2);
+    ph.g_n4 = (float)(d->omega0 / c * d->n4);
+    ph.Ui = d->Ui;
+    ph.half_sigma = d->sigma_drude / 2.0;
+    ph.w0tc = d->omega0 * d->tau_c;
+    // the reference passes these three as float32 scalars (media_physics.py:185-188)
+    ph.rho_nt = (double)(float)d->rho_nt;
+    ph.r_rec = (double)(float)(1.0 / d->tau_rec);
+    ph.dt = (double)(float)d->dt;
+    ph.av_coef = d->av_coef;
+    ph.lut_n = d->lut_n;
+    ph.lut_x0 = d->lut_x0;
+    ph.lut_x1 = d->lut_x1;
+    ph.lut_dx = (d->lut_x1 - d->lut_x0) / (double)(d->lut_n - 1);
+    // below the largest grid point whose rate and all lower rates round to 0 in float32,
+    // the interpolated rate is exactly 0 after the cast
+    int j0 = 0;
+    while (j0 < d->lut_n && (float)lut_w[j0] == 0.0f) ++j0;
+    ph.lut_zero_below = (j0 >= 2) ? pow(10.0, ph.lut_x0 + (j0 - 1) * ph.lut_dx) * (1.0 - 1e-12)
+                                  : -1.0;
+    if (j0 >= 2 && j0 >= d->lut_n) ph.lut_zero_below = 1e308;
+    {
+        const double G = -ph.r_rec;
+        double gdt = G * ph.dt;
+        gdt = gdt > 10.0 ? 10.0 : (gdt < -10.0 ? -10.0 : gdt);
+        ph.e_rec = exp(gdt);
+        ph.f_rec = fabs(gdt) < 1e-5 ? ph.dt * (1.0 + 0.5 * gdt) : (ph.e_rec - 1.0) / G;
+    }
+    ph.use_shock = d->use_shock ? 1 : 0;
+    (void)s;
+    *out = p;
+    return UPPE_OK;
+}
+
+int uppe3d_plan_destroy(uppe3d_plan *p) {
+    delete p;
+    return UPPE_OK;
+}
+
+int uppe3d_load_field(uppe3d_plan *p, const void *a0, void *stream) {
+    UPPE_CHECK(p && a0, "uppe3d_load_field: null argument");
+    hipStream_t s = (hipStream_t)stream;
+    const int nx = p->d.nx, ny = p->d.ny;
+    const float inv = (float)(1.0 / p->d.a_scale);
+    switch (p->d.nt) {
+#define LOAD(NT, LN)                                                                      \
+    case NT:                                                                              \
+        load_kernel<NT, LN><<<nx * (ny / LN), 64 * LN, 0, s>>>((const float2 *)a0, p->f, \
+                                                            p->tb, p->ctl, inv, nx, ny); \
+        break;
+        LOAD(128, 16)
+        LOAD(256, 16)
+        LOAD(512, 16)
+        LOAD(1024, 16)
+        LOAD(2048, 8)
+#undef LOAD
+        default:
+            return UPPE_EINVAL;
+    }
+    UPPE_HIP(hipGetLastError());
+    return UPPE_OK;
+}
+
+static int store_common(uppe3d_plan *p, void *out, void *stream, bool spectrum) {
+    UPPE_CHECK(p && out, "uppe3d_store: null argument");
+    hipStream_t s = (hipStream_t)stream;
+    const int nx = p->d.nx, ny = p->d.ny;
+    const float sc = (float)p->d.a_scale;
+    switch (p->d.nt) {
+#define STORE(NT, LN)                                                                         \
+    case NT:                                                                                  \
+        if (spectrum)                                                                         \
+            store_kernel<NT, LN, true><<<nx * (ny / LN), 64 * LN, 0, s>>>(                     \
+                (float2 *)out, p->f, p->tb, p->ctl, sc, nx, ny);                              \
+        else                                                                                  \
+            store_kernel<NT, LN, false><<<nx * (ny / LN), 64 * LN, 0, s>>>(                    \
+                (float2 *)out, p->f, p->tb, p->ctl, sc, nx, ny);                              \
+        break;
+        STORE(128, 16)
+        STORE(256, 16)
+        STORE(512, 16)
+        STORE(1024, 16)
+        STORE(2048, 8)
+#undef STORE
+        default:
+            return UPPE_EINVAL;
+    }
+    UPPE_HIP(hipGetLastError());
+    return UPPE_OK;
+}
+
+int uppe3d_store_spectrum(uppe3d_plan *p, void *out, void *stream) {
+    return store_common(p, out, stream, true);
+}
+int uppe3d_store_field(uppe3d_plan *p, void *out, void *stream) {
+    return store_common(p, out, stream, false);
+}
+
+int uppe3d_begin(uppe3d_plan *p, double L, double tol, double max_step, double min_step,
+                 void *stream) {
+    UPPE_CHECK(p != nullptr, "uppe3d_begin: null plan");
+    UPPE_CHECK(tol > 0.0 && max_step > 0.0 && min_step >= 0.0,
+               "uppe3d_begin: tol and max_step must be positive");
+    begin_kernel<<<1, 1, 0, (hipStream_t)stream>>>(p->ctl, L, tol, max_step, min_step);
+    UPPE_HIP(hipGetLastError());
+    return UPPE_OK;
+}
+
+int uppe3d_attempt(uppe3d_plan *p, void *stream) {
+    UPPE_CHECK(p != nullptr, "uppe3d_attempt: null plan");
+    return enqueue_attempt(p, (hipStream_t)stream);
+}
+
+int uppe3d_replay_attempt(uppe3d_plan *p, double h, int accept, void *stream) {
+    UPPE_CHECK(p != nullptr && h > 0.0, "uppe3d_replay_attempt: bad argument");
+    hipStream_t s = (hipStream_t)stream;
+    replay_set_kernel<<<1, 1, 0, s>>>(p->ctl, h, accept, h);
+    UPPE_HIP(hipGetLastError());
+    return enqueue_attempt(p, s);
+}
+
+int uppe3d_status_get(uppe3d_plan *p, void *stream, uppe3d_status *st) {
+    UPPE_CHECK(p && st, "uppe3d_status_get: null argument");
+    Ctl3D c;
+    UPPE_HIP(hipMemcpyAsync(&c, p->ctl, sizeof c, hipMemcpyDeviceToHost, (hipStream_t)stream));
+    UPPE_HIP(hipStreamSynchronize((hipStream_t)stream));
+    st->z = c.z;
+    st->h = c.h;
+    st->err = c.err;
+    st->attempts = c.attempts;
+    st->accepted = c.accepted;
+    st->rejected = c.rejected;
+    st->done = c.done;
+    st->nonfinite = c.nonfinite;
+    st->fail_count = c.fail_count;
+    st->cur = c.cur;
+    return UPPE_OK;
+}
+
+int uppe3d_run(uppe3d_plan *p, int max_attempts, int check_every, void *stream,
+               uppe3d_status *st) {
+    UPPE_CHECK(p && st, "uppe3d_run: null argument");
+    if (check_every < 1) check_every = 1;
+    int launched = 0;
+    RET(uppe3d_status_get(p, stream, st));
+    while (!st->done && (max_attempts <= 0 || launched < max_attempts)) {
+        int n = check_every;
+        if (max_attempts > 0 && launched + n > max_attempts) n = max_attempts - launched;
+        for (int i = 0; i < n; ++i) RET(enqueue_attempt(p, (hipStream_t)stream));
+        launched += n;
+        RET(uppe3d_status_get(p, stream, st));
+    }
+    if (st->nonfinite) {
+        uppe::set_error("uppe3d_run: error norm became non-finite at z = %.9g after %d attempts",
+                        st->z, st->attempts);
+        return UPPE_ENONFINITE;
+    }
+    return UPPE_OK;
+}
+
+int uppe3d_state_ptr(uppe3d_plan *p, void *stream, void **state) {
+    UPPE_CHECK(p && state, "uppe3d_state_ptr: null argument");
+    uppe3d_status st;
+    RET(uppe3d_status_get(p, stream, &st));
+    *state = st.cur ? p->f.B1 : p->f.B0;
+    return UPPE_OK;
+}
+
+}  // extern "C"

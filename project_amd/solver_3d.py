@@ -1,0 +1,230 @@
+"""3+1D envelope ERK4(3)-IP UPPE solver on MI355X (drop-in for
+``src/3+1D-UPPE-ERK4(3)IP-envelope/solver_3d.py:7-233``).
+
+Host side (FP64 NumPy, once per solver): grids, the medium's refractive-index hook, the
+per-omega pieces of the dispersion operator D = i (kz - k_ref) with the forward-difference k1
+of ``solver_3d.py:56-73``, absorber (``:89-98``), anti-alias filter (``:47-50``),
+valid-frequency mask (``:52-54``), shock factor and ionisation LUT.  Device side (libuppe.so,
+HIP): every ERK4(3)-IP attempt, its nonlinear evaluations, the L-infinity error norm and the
+step controller, with no host synchronisation inside an attempt.
+
+Call contract kept: ``propagate(A0, L, tol=1e-4, max_step=1e-2, min_step=1e-9,
+return_gpu=False) -> (z, freq_axis_hz, A_w)`` with A_w = fftshift_t(FFT_t(A(L))) of shape
+(Nx, Ny, Nt).  ``return_gpu=True`` returns a torch tensor on the GPU (CuPy does not exist on
+ROCm).  One difference: a non-finite error norm raises ``FloatingPointError`` (the
+reference's loop never terminates in that case); KeyboardInterrupt returns the partial
+field, as in the reference (``:210-211``).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+C_LIGHT = 299792458.0
+
+
+def _fftfreq(n, d):
+    return np.fft.fftfreq(n, d)
+
+
+class ERK43IP_UPPE_3D_Optimized:
+    def __init__(self, x_grid, y_grid, t_grid, medium, precision="single"):
+        import torch
+
+        if hasattr(medium, "calc_nonlinear_response"):
+            raise NotImplementedError(
+                "medium defines calc_nonlinear_response; the fused HIP kernel computes the "
+                "OpticalMedium response from the medium's parameters only")
+        if getattr(medium, "use_raman", False):
+            raise NotImplementedError("Raman response is not implemented in the 3D kernel")
+        self.medium = medium
+        self.c = medium.c
+        self.lambda0 = medium.lambda0
+        self.omega0 = medium.omega0
+        self.precision = precision
+        self.float_dtype = np.float32
+        self.complex_dtype = np.complex64
+        x = np.asarray(x_grid, np.float64)
+        y = np.asarray(y_grid, np.float64)
+        t = np.asarray(t_grid, np.float64)
+        self.x, self.y, self.t = x, y, t
+        self.Nx, self.Ny, self.Nt = len(x), len(y), len(t)
+        self.dx, self.dy, self.dt = float(x[1] - x[0]), float(y[1] - y[0]), float(t[1] - t[0])
+        kx = 2 * np.pi * _fftfreq(self.Nx, self.dx)
+        ky = 2 * np.pi * _fftfreq(self.Ny, self.dy)
+        self.kx2, self.ky2 = kx ** 2, ky ** 2
+        omega = 2 * np.pi * _fftfreq(self.Nt, self.dt)
+        self.omega = omega
+        self.freq_axis_hz = np.fft.fftshift(omega) / (2 * np.pi)
+        self._precompute_dispersion_operator()
+        self._init_absorber()
+        self.shock_operator = np.asarray(medium.get_shock_operator(omega), np.float64).reshape(-1)
+        f = omega / (2 * np.pi)
+        fmax = float(np.max(np.abs(f))) + 1e-10
+        self.anti_alias_filter = np.exp(-(f / (0.85 * fmax)) ** 16)
+        self.omega_abs = omega + self.omega0
+        self.valid_freq_mask = (self.omega_abs > 0).astype(np.float64)
+        self.A_scale = 1.0
+        self.device = torch.device("cuda")
+        self._plan = None
+        self._ws = None
+        self.last_status = None
+        self.step_log = None
+
+    # -- host setup (FP64) -----------------------------------------------------------------
+    def _precompute_dispersion_operator(self):
+        """Per-omega k, k_ref (forward-difference k1, dw = omega0 * 1e-3) of solver_3d.py:56-87."""
+        m, w0 = self.medium, self.omega0
+        n_w = np.asarray(m.refractive_index(self.omega + w0), np.float64)
+        self.k_w = n_w * (self.omega + w0) / C_LIGHT
+        n_c = float(np.asarray(m.refractive_index(np.array([w0])))[0])
+        dw = w0 * 1e-3
+        n_p = float(np.asarray(m.refractive_index(np.array([w0 + dw])))[0])
+        k0 = n_c * w0 / C_LIGHT
+        k1 = (n_p * (w0 + dw) / C_LIGHT - k0) / dw
+        self.k0, self.k1 = k0, k1
+        self.kref_w = k0 + k1 * self.omega
+        self.valid_w = ((self.omega + w0) > 0).astype(np.float64)
+
+    def dispersion_operator(self):
+        """The full D(x, y, omega) in FP64 (host; for inspection and tests)."""
+        kp = (self.kx2[:, None] + self.ky2[None, :])[:, :, None]
+        ksq = (self.k_w ** 2)[None, None, :]
+        kr = self.kref_w[None, None, :]
+        kz = np.sqrt((ksq - kp).astype(np.complex128))
+        return 1j * ((ksq - kr ** 2) - kp) / (kz + kr) * self.valid_w[None, None, :]
+
+    def _init_absorber(self):
+        X, Y = np.meshgrid(self.x, self.y, indexing="ij")
+        R = np.sqrt(X ** 2 + Y ** 2)
+        hw = float(np.max(np.abs(self.x)))
+        s = np.clip((R - 0.85 * hw) / (0.98 * hw - 0.85 * hw + 1e-30), 0.0, 1.0)
+        self.damp_space = 120.0 * s ** 10
+
+    def _lut_arrays(self):
+        m = self.medium
+        lg = np.ascontiguousarray(m.log10_I_cpu, np.float64)
+        W = np.ascontiguousarray(m.W_cpu, np.float64)
+        step = np.diff(lg)
+        if lg.size < 2 or not np.allclose(step, step[0], rtol=1e-9, atol=0):
+            raise NotImplementedError("ionisation LUT must be evenly spaced in log10(I)")
+        return lg, W
+
+    def _make_plan(self, a_scale):
+        import torch
+        lib = _lib.load()
+        m = self.medium
+        lg, W = self._lut_arrays()
+        d = _lib.Uppe3DDesc(
+            nx=self.Nx, ny=self.Ny, nt=self.Nt, dt=self.dt, omega0=self.omega0,
+            a_scale=a_scale, n2=float(m.n2), n4=float(getattr(m, "n4", 0.0)), Ui=float(m.Ui),
+            rho_nt=float(m.rho_nt), tau_rec=float(m.tau_rec), tau_c=float(m.tau_c),
+            sigma_drude=float(m.sigma_drude), av_coef=float(m.avalanche_coefficient()),
+            use_shock=int(bool(m.use_shock)), lut_n=int(lg.size), lut_x0=float(lg[0]),
+            lut_x1=float(lg[-1]))
+        nbytes = ctypes.c_size_t(0)
+        _lib.check(lib.uppe3d_workspace_size(ctypes.byref(d), ctypes.byref(nbytes)),
+                   "uppe3d_workspace_size")
+        if self._ws is None or self._ws.numel() < nbytes.value:
+            self._ws = torch.empty(nbytes.value, dtype=torch.uint8, device=self.device)
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (
+            self.k_w, self.kref_w, self.valid_w, self.anti_alias_filter, self.shock_operator,
+            self.valid_freq_mask, self.kx2, self.ky2, self.damp_space.reshape(-1), W)]
+        if self._plan is not None:
+            lib.uppe3d_plan_destroy(self._plan)
+            self._plan = None
+        plan = ctypes.c_void_p()
+        _lib.check(lib.uppe3d_plan_create(
+            ctypes.byref(d), _lib.ptr(self._ws), nbytes, *[_lib.host_ptr(a) for a in arrs],
+            _lib.stream_handle(), ctypes.byref(plan)), "uppe3d_plan_create")
+        self._plan = plan
+        self._desc = d
+
+    def __del__(self):
+        try:
+            if self._plan is not None:
+                _lib.load().uppe3d_plan_destroy(self._plan)
+        except Exception:
+            pass
+
+    # -- propagation -----------------------------------------------------------------------
+    def _to_device(self, A0):
+        import torch
+        if isinstance(A0, torch.Tensor):
+            a = A0.to(device=self.device, dtype=torch.complex64)
+        else:
+            a = torch.from_numpy(np.ascontiguousarray(A0, np.complex64)).to(self.device)
+        if tuple(a.shape) != (self.Nx, self.Ny, self.Nt):
+            raise ValueError("A0 must have shape (Nx, Ny, Nt) = %s" % ((self.Nx, self.Ny, self.Nt),))
+        return a.contiguous()
+
+    def load(self, A0):
+        """Uploads A0 and prepares the plan (A_scale = max|A0|, solver_3d.py:163-170)."""
+        lib = _lib.load()
+        a = self._to_device(A0)
+        s = float(a.abs().max().item())
+        self.A_scale = s if s >= 1e-30 else 1.0
+        self._make_plan(self.A_scale)
+        _lib.check(lib.uppe3d_load_field(self._plan, _lib.ptr(a), _lib.stream_handle()),
+                   "uppe3d_load_field")
+        return a
+
+    def status(self):
+        st = _lib.Uppe3DStatus()
+        _lib.check(_lib.load().uppe3d_status_get(self._plan, _lib.stream_handle(),
+                                                 ctypes.byref(st)), "uppe3d_status_get")
+        return st
+
+    def propagate(self, A0, L, tol=1e-4, max_step=1e-2, min_step=1e-9, return_gpu=False,
+                  check_every=8, max_attempts=0):
+        lib = _lib.load()
+        self.load(A0)
+        s = _lib.stream_handle()
+        _lib.check(lib.uppe3d_begin(self._plan, L, tol, max_step, min_step, s), "uppe3d_begin")
+        st = _lib.Uppe3DStatus()
+        try:
+            rc = lib.uppe3d_run(self._plan, int(max_attempts), int(check_every), s,
+                                ctypes.byref(st))
+            if rc == -3:
+                raise FloatingPointError(lib.uppe_last_error().decode())
+            _lib.check(rc, "uppe3d_run")
+        except KeyboardInterrupt:
+            st = self.status()
+            print("\n[Solver] Interrupted at z=%.3f mm" % (st.z * 1000))
+        self.last_status = self.status()
+        return self._finish(return_gpu)
+
+    def propagate_replay(self, A0, steps, return_gpu=False):
+        """Replays a step sequence [(h, err, accepted), ...] (e.g. an oracle log)."""
+        lib = _lib.load()
+        self.load(A0)
+        s = _lib.stream_handle()
+        _lib.check(lib.uppe3d_begin(self._plan, 1e300, 1.0, 1.0, 0.0, s), "uppe3d_begin")
+        errs = []
+        for (h, _, acc) in steps:
+            _lib.check(lib.uppe3d_replay_attempt(self._plan, float(h), int(bool(acc)), s),
+                       "uppe3d_replay_attempt")
+            errs.append(self.status().err)
+        self.last_status = self.status()
+        self.replay_errors = errs
+        return self._finish(return_gpu)
+
+    def _finish(self, return_gpu):
+        import torch
+        lib = _lib.load()
+        out = torch.empty((self.Nx, self.Ny, self.Nt), dtype=torch.complex64, device=self.device)
+        _lib.check(lib.uppe3d_store_spectrum(self._plan, _lib.ptr(out), _lib.stream_handle()),
+                   "uppe3d_store_spectrum")
+        z = float(self.last_status.z)
+        if return_gpu:
+            return z, self.freq_axis_hz, out
+        return z, self.freq_axis_hz, out.cpu().numpy()
+
+    def field(self):
+        """Current field A(x, y, t) (physical units) as a torch tensor on the GPU."""
+        import torch
+        out = torch.empty((self.Nx, self.Ny, self.Nt), dtype=torch.complex64, device=self.device)
+        _lib.check(_lib.load().uppe3d_store_field(self._plan, _lib.ptr(out),
+                                                  _lib.stream_handle()), "uppe3d_store_field")
+        return out
