@@ -1,0 +1,158 @@
+#!/usr/bin/env python
+"""Benchmark of the 3+1D envelope ERK4(3)-IP UPPE propagator on one MI355X.
+
+Workload: argon at 1 bar (the reference Argon medium: Kerr n2, HOKE n4, LUT ionisation,
+FP64 plasma recurrence, with the tunnel exponent evaluated without the reference's 1e-20
+guard), Gaussian beam w0 = 1 mm, 40 fs, 800 nm, 0.8 mJ with seeded 1e-4 spatial noise, grid
+128 x 128 x 1024 complex64, adaptive tolerance 1e-5 over L = 0.5 m.  A "step" is one
+ERK4(3)-IP attempt (accepted or rejected) of the adaptive propagation: 5 nonlinear
+evaluations, the L-infinity error norm and the on-device controller.  The field is resident in
+HBM when the timed region starts.
+
+Prints one JSON line: z-steps/s, per-kernel HBM roofline (HIP events around every launch of
+the timed attempts), and the FP64 oracle timed on the host cores on a bounded sample.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# launch slots of one attempt and their algorithmic complex64-field transfers
+SLOTS = ["T1", "P2", "T2", "P3", "T3", "P4", "T4", "P5", "T5", "P6", "CTL"]
+FIELD_PASSES = {"T1": 2, "P2": 5, "T2": 2, "P3": 5, "T3": 2, "P4": 5, "T4": 2, "P5": 3,
+                "T5": 4, "P6": 1, "CTL": 0}
+GROUPS = {"tpass": ["T1", "T2", "T3", "T4"], "tpass_final": ["T5"], "xy_stage": ["P2", "P4"],
+          "xy_stage_noexp": ["P3"], "xy_final": ["P5"], "xy_err": ["P6"], "ctl": ["CTL"]}
+
+
+def cpu_baseline(nx, nt, attempts, workers):
+    """FP64 oracle (NumPy/SciPy restatement of the reference) on the same workload."""
+    from oracle import uppe3d as o3
+    from project_amd import workloads as wl
+    x, y, t = wl.grid3d(nx, nt)
+    A0 = wl.beam3d(x, y, t)
+    med = o3.GasMedium("Argon", 800e-9, 1e5 / 101325, tunnel_guard=0.0)
+    orc = o3.Oracle3D(x, y, t, med, workers=workers)
+    t0 = time.perf_counter()
+    z, f, Aw, log = orc.propagate(A0, 0.5, tol=1e-5, max_attempts=attempts)
+    dt = time.perf_counter() - t0
+    return len(log) / dt, len(log), dt
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--nx", type=int, default=128)
+    ap.add_argument("--nt", type=int, default=1024)
+    ap.add_argument("--cpu-attempts", type=int, default=1,
+                    help="oracle attempts timed for the CPU baseline (0 skips it)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the 128x128x1024 field is not "
+              "sharded; see the sharding design notes). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from project_amd import _lib, media_physics as mp, workloads as wl
+    from project_amd.solver_3d import ERK43IP_UPPE_3D_Optimized
+
+    lib = _lib.load()
+    nx, nt = args.nx, args.nt
+    x, y, t = wl.grid3d(nx, nt)
+    A0 = wl.beam3d(x, y, t)
+    med = mp.Argon(800e-9, pressure_atm=1e5 / 101325, tunnel_guard=0.0)
+    solver = ERK43IP_UPPE_3D_Optimized(x, y, t, med)
+    solver.load(A0)
+    s = _lib.stream_handle()
+    _lib.check(lib.uppe3d_begin(solver._plan, 0.5, 1e-5, 1e-2, 1e-9, s), "uppe3d_begin")
+    for _ in range(args.warmup):
+        _lib.check(lib.uppe3d_attempt(solver._plan, s), "uppe3d_attempt")
+    torch.cuda.synchronize()
+    st0 = solver.status()
+
+    kms = (ctypes.c_float * 11)()
+    tot = ctypes.c_float(0.0)
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    _lib.check(lib.uppe3d_attempt_timed(solver._plan, args.steps, s, kms, ctypes.byref(tot)),
+               "uppe3d_attempt_timed")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    st1 = solver.status()
+    if st1.done:
+        raise RuntimeError("propagation finished inside the timed region; lower --steps")
+    ms_event = float(tot.value)
+    ms_per_step = ms_event / args.steps
+    steps_per_s = 1000.0 / ms_per_step
+
+    field_bytes = 8.0 * nx * nx * nt
+    kernels = {}
+    for g, slots in GROUPS.items():
+        ms = sum(kms[SLOTS.index(k)] for k in slots)
+        launches = len(slots) * args.steps
+        byts = FIELD_PASSES[slots[0]] * field_bytes
+        avg_ms = ms / launches
+        kernels[g] = {"avg_ms": avg_ms, "share": ms / ms_event,
+                      "bytes_per_launch": byts,
+                      "GBps": (byts / (avg_ms * 1e-3) / 1e9) if byts else None}
+    dom = max((g for g in kernels if kernels[g]["bytes_per_launch"] > 0),
+              key=lambda g: kernels[g]["share"])
+    d = kernels[dom]
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(d["GBps"], 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(d["GBps"] / HBM_PEAK_GBS, 4),
+                "traffic": None}
+    step_gbs_37 = 37 * field_bytes * steps_per_s / 1e9
+    step_gbs_31 = 31 * field_bytes * steps_per_s / 1e9
+
+    cpu = None
+    if args.cpu_attempts > 0:
+        workers = min(os.cpu_count() or 1, 16)
+        v, n, secs = cpu_baseline(nx, nt, args.cpu_attempts, workers)
+        cpu = {"value": round(v, 5), "unit": "z-steps/s", "cores": workers, "kind": "port",
+               "sample": "%d ERK4(3)-IP attempt(s) of the same workload from z=0 with the FP64 "
+                         "oracle (NumPy/SciPy pocketfft, scipy workers=%d), %.1f s"
+                         % (n, workers, secs)}
+
+    out = {
+        "metric": "3+1D UPPE z-steps/s (128^2x1024) + HBM GB/s",
+        "value": round(steps_per_s, 2),
+        "unit": "z-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (complex64; FP64 plasma recurrence and error norm)",
+        "data": "synthetic (seeded Gaussian beam + 1e-4 spatial noise)",
+        "config": {"workload": "3+1D envelope UPPE ERK4(3)-IP, argon 1 bar (Kerr+HOKE+LUT "
+                               "plasma), 0.8 mJ 40 fs 800 nm, w0 1 mm, tol 1e-5, L 0.5 m",
+                   "grid": [nx, nx, nt], "z_start": st0.z, "z_end": st1.z,
+                   "attempts": st1.attempts - st0.attempts,
+                   "accepted": st1.accepted - st0.accepted,
+                   "wall_s": round(wall, 4)},
+        "roofline": roofline,
+        "step_hbm_GBps": {"model_37F": round(step_gbs_37, 1), "model_31F": round(step_gbs_31, 1),
+                          "frac_37F": round(step_gbs_37 / HBM_PEAK_GBS, 4)},
+        "kernels": {g: {k: (round(v, 5) if isinstance(v, float) else v) for k, v in kd.items()}
+                    for g, kd in kernels.items()},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = round(steps_per_s / cpu["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

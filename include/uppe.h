@@ -84,6 +84,12 @@ int uppe3d_begin(uppe3d_plan *p, double L, double tol, double max_step, double m
                  void *stream);
 /* One ERK4(3)-IP attempt with the on-device controller (no host synchronisation). */
 int uppe3d_attempt(uppe3d_plan *p, void *stream);
+/* n attempts as uppe3d_attempt, with a HIP event recorded between consecutive launches;
+ * synchronises once at the end.  kernel_ms[11] receives each launch slot's summed duration in
+ * launch order T1 P2 T2 P3 T3 P4 T4 P5 T5 P6 CTL (Tn: fused t-passes = N evaluations, Pn:
+ * xy-plane passes); total_ms the time from the first to the last event. */
+int uppe3d_attempt_timed(uppe3d_plan *p, int n, void *stream, float *kernel_ms,
+                         float *total_ms);
 /* One attempt with a prescribed step h and accept decision (step-sequence replay). */
 int uppe3d_replay_attempt(uppe3d_plan *p, double h, int accept, void *stream);
 /* Runs attempts until done or max_attempts, reading the status every `check_every` attempts. */

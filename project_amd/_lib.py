@@ -54,6 +54,8 @@ SIGNATURES = {
     "uppe3d_load_field": (c_int, [c_void_p, c_void_p, c_void_p]),
     "uppe3d_begin": (c_int, [c_void_p, c_double, c_double, c_double, c_double, c_void_p]),
     "uppe3d_attempt": (c_int, [c_void_p, c_void_p]),
+    "uppe3d_attempt_timed": (c_int, [c_void_p, c_int, c_void_p, P(ctypes.c_float),
+                                     P(ctypes.c_float)]),
     "uppe3d_replay_attempt": (c_int, [c_void_p, c_double, c_int, c_void_p]),
     "uppe3d_run": (c_int, [c_void_p, c_int, c_int, c_void_p, P(Uppe3DStatus)]),
     "uppe3d_status_get": (c_int, [c_void_p, c_void_p, P(Uppe3DStatus)]),

@@ -873,20 +873,39 @@ int launch_xy(uppe3d_plan *p, int mode, hipStream_t s) {
         if (r_ != UPPE_OK) return r_; \
     } while (0)
 
-int enqueue_attempt(uppe3d_plan *p, hipStream_t s) {
+// One attempt: 11 launches.  With `ev` (12 events) an event is recorded before the first
+// and after every launch, so the caller can read per-kernel durations.
+int enqueue_attempt(uppe3d_plan *p, hipStream_t s, hipEvent_t *ev = nullptr) {
     Fields3D &f = p->f;
+    int k = 0;
+    auto mark = [&]() -> int {
+        if (ev) UPPE_HIP(hipEventRecord(ev[k++], s));
+        return UPPE_OK;
+    };
+    RET(mark());
     RET(launch_tpass(p, 0, nullptr, 1, f.Yw, s));   // T1 on the state
+    RET(mark());
     RET(launch_xy(p, 2, s));                          // P2
+    RET(mark());
     RET(launch_tpass(p, 0, f.X, 0, f.Yw, s));        // T2
+    RET(mark());
     RET(launch_xy(p, 3, s));                          // P3
+    RET(mark());
     RET(launch_tpass(p, 0, f.X, 0, f.Yw, s));        // T3
+    RET(mark());
     RET(launch_xy(p, 4, s));                          // P4
+    RET(mark());
     RET(launch_tpass(p, 0, f.X, 0, f.Y4, s));        // T4
+    RET(mark());
     RET(launch_xy(p, 5, s));                          // P5
+    RET(mark());
     RET(launch_tpass(p, 1, f.X, 0, f.Yw, s));        // T5 (D and next state)
+    RET(mark());
     RET(launch_xy(p, 6, s));                          // P6
+    RET(mark());
     ctl_kernel<<<1, 1, 0, s>>>(p->ctl);
     UPPE_HIP(hipGetLastError());
+    RET(mark());
     return UPPE_OK;
 }
 
@@ -1096,6 +1115,38 @@ int uppe3d_begin(uppe3d_plan *p, double L, double tol, double max_step, double m
 int uppe3d_attempt(uppe3d_plan *p, void *stream) {
     UPPE_CHECK(p != nullptr, "uppe3d_attempt: null plan");
     return enqueue_attempt(p, (hipStream_t)stream);
+}
+
+int uppe3d_attempt_timed(uppe3d_plan *p, int n, void *stream, float *kernel_ms,
+                         float *total_ms) {
+    UPPE_CHECK(p && kernel_ms && total_ms && n > 0, "uppe3d_attempt_timed: bad argument");
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<hipEvent_t> ev(12 * (size_t)n);
+    int rc = UPPE_OK;
+    size_t made = 0;
+    for (; made < ev.size(); ++made)
+        if (hipEventCreate(&ev[made]) != hipSuccess) {
+            rc = UPPE_EHIP;
+            uppe::set_error("uppe3d_attempt_timed: hipEventCreate failed");
+            break;
+        }
+    for (int a = 0; a < n && rc == UPPE_OK; ++a) rc = enqueue_attempt(p, s, &ev[12 * (size_t)a]);
+    if (rc == UPPE_OK && hipEventSynchronize(ev.back()) != hipSuccess) {
+        rc = UPPE_EHIP;
+        uppe::set_error("uppe3d_attempt_timed: hipEventSynchronize failed");
+    }
+    if (rc == UPPE_OK) {
+        for (int i = 0; i < 11; ++i) kernel_ms[i] = 0.0f;
+        for (int a = 0; a < n; ++a)
+            for (int i = 0; i < 11; ++i) {
+                float ms = 0.0f;
+                hipEventElapsedTime(&ms, ev[12 * (size_t)a + i], ev[12 * (size_t)a + i + 1]);
+                kernel_ms[i] += ms;
+            }
+        hipEventElapsedTime(total_ms, ev[0], ev.back());
+    }
+    for (size_t i = 0; i < made; ++i) hipEventDestroy(ev[i]);
+    return rc;
 }
 
 int uppe3d_replay_attempt(uppe3d_plan *p, double h, int accept, void *stream) {
