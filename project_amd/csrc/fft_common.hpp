@@ -32,16 +32,26 @@ __device__ __forceinline__ float2 mul_mi(float2 a) {
     return DIR > 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
 }
 
-// cos / sin of pi*k/8, k = 0..7 (exact decimal expansions rounded to float)
-__device__ __forceinline__ float c16(int k) {
-    const float t[8] = {1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f,
-                        0.0f, -0.38268343236508977f, -0.70710678118654752f, -0.92387953251128674f};
-    return t[k];
+// cos / sin of pi*m/16, m = 0..15 (decimal expansions rounded to float)
+__device__ __forceinline__ float c32(int m) {
+    const float t[16] = {1.0f, 0.98078528040323044913f, 0.92387953251128675613f,
+                         0.83146961230254523708f, 0.70710678118654752440f,
+                         0.55557023301960222474f, 0.38268343236508977173f,
+                         0.19509032201612826785f, 0.0f, -0.19509032201612826785f,
+                         -0.38268343236508977173f, -0.55557023301960222474f,
+                         -0.70710678118654752440f, -0.83146961230254523708f,
+                         -0.92387953251128675613f, -0.98078528040323044913f};
+    return t[m];
 }
-__device__ __forceinline__ float s16(int k) {
-    const float t[8] = {0.0f, 0.38268343236508977f, 0.70710678118654752f, 0.92387953251128674f,
-                        1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f};
-    return t[k];
+__device__ __forceinline__ float s32(int m) {
+    const float t[16] = {0.0f, 0.19509032201612826785f, 0.38268343236508977173f,
+                         0.55557023301960222474f, 0.70710678118654752440f,
+                         0.83146961230254523708f, 0.92387953251128675613f,
+                         0.98078528040323044913f, 1.0f, 0.98078528040323044913f,
+                         0.92387953251128675613f, 0.83146961230254523708f,
+                         0.70710678118654752440f, 0.55557023301960222474f,
+                         0.38268343236508977173f, 0.19509032201612826785f};
+    return t[m];
 }
 
 // In-place DFT of R values held in registers, natural order in and out.
@@ -59,10 +69,10 @@ struct Dft {
         Dft<R / 2, DIR>::run(o);
 #pragma unroll
         for (int k = 0; k < R / 2; ++k) {
-            // W_R^k = exp(-2 pi i k / R) = cos(pi*(16k/R)/8) - i sin(...)
-            const int m = k * (16 / R);
-            float2 w = make_float2(c16(m & 7) * ((m & 8) ? -1.f : 1.f),
-                                   -DIR * s16(m & 7) * ((m & 8) ? -1.f : 1.f));
+            // W_R^k = exp(-2 pi i k / R) = cos(pi m / 16) - i sin(pi m / 16), m = 32 k / R
+            static_assert(R <= 32, "register DFT supports R <= 32");
+            const int m = k * (32 / R);
+            float2 w = make_float2(c32(m), -DIR * s32(m));
             float2 t = (k == 0) ? o[k] : cmul(o[k], w);
             v[k] = cadd(e[k], t);
             v[k + R / 2] = csub(e[k], t);

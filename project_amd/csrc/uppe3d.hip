@@ -81,6 +81,8 @@ struct Phys3D {
     double lut_x0, lut_dx, lut_x1, lut_zero_below;
     double e_rec, f_rec;        // exp(gdt) and step factor for Gamma = -r_rec
     int lut_n, use_shock;
+    int lut_bound_ok;           // W and W/I nondecreasing above a zero region (host-checked)
+    int pad1;
 };
 
 struct Fields3D {
@@ -132,7 +134,19 @@ struct LineFFT<N, EPT, N, DIR> {
 // ---------------------------------------------------------------------------------------
 // Ionisation rate: np.interp on the log10(I) grid in FP64, then the float32 cast of
 // media_physics.py:131-137,169.
-__device__ __forceinline__ float lut_rate(float I, const double *__restrict__ lut,
+__device__ __noinline__ float lut_rate(float I, const double *__restrict__ lut,
+                                      const Phys3D &ph) {
+    double x = log10((double)I + 1e-30);
+    x = fmin(fmax(x, ph.lut_x0), ph.lut_x1);
+    int j = (int)((x - ph.lut_x0) / ph.lut_dx);
+    j = j < 0 ? 0 : (j > ph.lut_n - 2 ? ph.lut_n - 2 : j);
+    const double xj = ph.lut_x0 + j * ph.lut_dx;
+    const double xj1 = (j + 1 == ph.lut_n - 1) ? ph.lut_x1 : ph.lut_x0 + (j + 1) * ph.lut_dx;
+    const double f0 = lut[j], f1 = lut[j + 1];
+    return (float)((f1 - f0) / (xj1 - xj) * (x - xj) + f0);
+}
+
+__device__ __forceinline__ float lut_rate_inl(float I, const double *__restrict__ lut,
                                           const Phys3D &ph) {
     double x = log10((double)I + 1e-30);
     x = fmin(fmax(x, ph.lut_x0), ph.lut_x1);
@@ -144,28 +158,146 @@ __device__ __forceinline__ float lut_rate(float I, const double *__restrict__ lu
     return (float)((f1 - f0) / (xj1 - xj) * (x - xj) + f0);
 }
 
-// Plasma coefficients of one line held by a wavefront, lane owning t = lane*EPT + i.
-// Plasma term = -(c1 + i c2) A with c1 = gen Ui/(2I + 1e-30) + sigma/2 rho,
-// c2 = sigma/2 rho omega0 tau_c (media_physics.py:198-203).  rho/gen follow the recurrence of
-// media_physics.py:11-66 in FP64: an exact affine prefix scan across the wave when the
-// avalanche rate is zero and no clamp can act (w*f <= exp(gdt) for every sample); otherwise
-// lane 0 walks the line serially through the LDS line buffer.
-template <int NT, int EPT>
-__device__ void plasma_line(const float (&I)[EPT], const float (&wi)[EPT], float (&c1)[EPT],
-                            float (&c2)[EPT], float2 *line, int lane, const Phys3D &ph) {
-    const bool aval = ph.av_coef != 0.0;
+// ---------------------------------------------------------------------------------------
+// Register-resident t-line FFT.  A wavefront holds one line of NT = 64 R samples, R per lane.
+// Four-step split NT = 64 x R: n = l + 64 r (l: lane group, r: register), k = k1 + R k2.
+//   omega -> t (inverse): lane at position lambda holds x[l + 64 r] with l = bitrev6(lambda);
+//     R-point DFT over registers, twiddle W_NT^(l k1), 64-point radix-2 DIT across lanes
+//     (bit-reversed positions in, natural out) -> lane lambda holds t = k1 + R lambda.
+//   t -> omega (forward): the transpose: radix-2 DIF across lanes (natural in, bit-reversed
+//     out), twiddle, R-point DFT -> lane lambda holds omega = bitrev6(lambda) + 64 r.
+__device__ __forceinline__ float2 shfl_xor2(float2 v, int m) {
+    return make_float2(__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64));
+}
+
+template <int DIR, int R>
+__device__ __forceinline__ void lanes_dit(float2 (&x)[R], int lane, const float2 *__restrict__ tw,
+                                          int nt) {
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+        float2 w = tw[(lane & (m - 1)) * (nt / (2 * m))];
+        if (DIR < 0) w.y = -w.y;
+        const bool hi = (lane & m) != 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const float2 t = hi ? cmul(x[r], w) : x[r];
+            const float2 p = shfl_xor2(t, m);
+            x[r] = hi ? csub(p, t) : cadd(t, p);
+        }
+    }
+}
+
+template <int DIR, int R>
+__device__ __forceinline__ void lanes_dif(float2 (&x)[R], int lane, const float2 *__restrict__ tw,
+                                          int nt) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        float2 w = tw[(lane & (m - 1)) * (nt / (2 * m))];
+        if (DIR < 0) w.y = -w.y;
+        const bool hi = (lane & m) != 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const float2 p = shfl_xor2(x[r], m);
+            x[r] = hi ? cmul(csub(p, x[r]), w) : cadd(x[r], p);
+        }
+    }
+}
+
+// x[r] = X(omega = l + 64 r), l = bitrev6(lane)  ->  x[k] = A(t = k + R lane), scaled by 1/NT
+template <int R>
+__device__ __forceinline__ void wave_ifft_t(float2 (&x)[R], int lane, int rl,
+                                            const float2 *__restrict__ tw) {
+    constexpr int NT = 64 * R;
+    Dft<R, -1>::run(x);
+#pragma unroll
+    for (int k = 1; k < R; ++k) x[k] = cmulc(x[k], tw[rl * k]);
+    lanes_dit<-1, R>(x, lane, tw, NT);
+    const float s = 1.0f / NT;
+#pragma unroll
+    for (int k = 0; k < R; ++k) x[k] = cscale(x[k], s);
+}
+
+// x[k] = y(t = k + R lane)  ->  x[r] = Y(omega = l + 64 r), l = bitrev6(lane), unnormalised
+template <int R>
+__device__ __forceinline__ void wave_fft_t(float2 (&x)[R], int lane, int rl,
+                                           const float2 *__restrict__ tw) {
+    constexpr int NT = 64 * R;
+    lanes_dif<1, R>(x, lane, tw, NT);
+#pragma unroll
+    for (int k = 1; k < R; ++k) x[k] = cmul(x[k], tw[rl * k]);
+    Dft<R, 1>::run(x);
+}
+
+// One step of the reference recurrence (media_physics.py:36-63) from rho; returns the
+// plasma coefficients (c1, c2) of the sample and advances rho.  k1_affine: avalanche rate
+// zero, so Gamma = -1/tau_rec and exp(gdt), the step factor are constants.
+__device__ __noinline__ void k1_affine(double &rho, double w, double I, const Phys3D &ph,
+                                      float &c1, float &c2) {
+    double depl = (ph.rho_nt - rho) / ph.rho_nt;
+    if (depl < 0.0) depl = 0.0;
+    const double gen = (w * ph.rho_nt) * depl;
+    const double cl = gen * ph.Ui / (2.0 * I + 1e-30);
+    const double cd = ph.half_sigma * rho;
+    c1 = (float)(cl + cd);
+    c2 = (float)(cd * ph.w0tc);
+    rho = rho * ph.e_rec + (w * ph.rho_nt * depl) * ph.f_rec;
+    rho = rho > ph.rho_nt ? ph.rho_nt : (rho < 0.0 ? 0.0 : rho);
+}
+
+__device__ __noinline__ void k1_general(double &rho, double w, double wav, double I,
+                                        const Phys3D &ph, float &c1, float &c2) {
+    double depl = (ph.rho_nt - rho) / ph.rho_nt;
+    if (depl < 0.0) depl = 0.0;
+    const double gen = (w * ph.rho_nt + wav * rho) * depl;
+    const double S = w * ph.rho_nt * depl;
+    const double G = wav * depl - ph.r_rec;
+    double gdt = G * ph.dt;
+    gdt = gdt > 10.0 ? 10.0 : (gdt < -10.0 ? -10.0 : gdt);
+    const double e = exp(gdt);
+    const double f = fabs(gdt) < 1e-5 ? ph.dt * (1.0 + 0.5 * gdt) : (e - 1.0) / G;
+    const double cl = gen * ph.Ui / (2.0 * I + 1e-30);
+    const double cd = ph.half_sigma * rho;
+    c1 = (float)(cl + cd);
+    c2 = (float)(cd * ph.w0tc);
+    rho = rho * e + S * f;
+    rho = rho > ph.rho_nt ? ph.rho_nt : (rho < 0.0 ? 0.0 : rho);
+}
+
+// register-array element by runtime index without dynamic indexing (keeps arrays in VGPRs)
+template <int R, class T>
+__device__ __forceinline__ T pick(const T (&a)[R], int i) {
+    T v = a[0];
+#pragma unroll
+    for (int k = 1; k < R; ++k) v = (i == k) ? a[k] : v;
+    return v;
+}
+
+// Plasma coefficients for a line whose lane holds t = k + R lane (time order = lane order),
+// delivered in time order to sink(k, c1, c2) with plasma term = -(c1 + i c2) A
+// (media_physics.py:198-203).  The density recurrence is an exact FP64 affine prefix scan
+// across the wave when the avalanche rate is zero and no clamp can act (w f <= exp(gdt) at
+// every sample); otherwise the lanes run the serial recurrence in turn, handing rho on.
+// sink(k, ...) may overwrite x[k]; it is called after sample k has been read.
+template <int R, class Sink>
+__device__ __forceinline__ void plasma_walk(float2 (&x)[R], int lane, const Phys3D &ph,
+                                            const double *__restrict__ lut, Sink sink) {
+    float wi[R];
     bool risk = false;
 #pragma unroll
-    for (int i = 0; i < EPT; ++i) risk |= ((double)wi[i] * ph.f_rec > ph.e_rec);
-    const bool serial = aval || (__ballot(risk) != 0);
-    if (!serial) {
+    for (int k = 0; k < R; ++k) {
+        const float I = ph.a_scale2 * cabs2(x[k]);
+        wi[k] = ((double)I > ph.lut_zero_below) ? lut_rate(I, lut, ph) : 0.0f;
+        risk |= ((double)wi[k] * ph.f_rec > ph.e_rec);
+    }
+    const bool aval = ph.av_coef != 0.0;
+    if (!aval && __ballot(risk) == 0) {
         double A = 1.0, Bc = 0.0;
 #pragma unroll
-        for (int i = 0; i < EPT; ++i) {
-            const double w = (double)wi[i];
-            const double ai = ph.e_rec - w * ph.f_rec;
-            Bc = ai * Bc + w * ph.rho_nt * ph.f_rec;
-            A = ai * A;
+        for (int k = 0; k < R; ++k) {
+            const double w = (double)wi[k];
+            const double ak = ph.e_rec - w * ph.f_rec;
+            Bc = ak * Bc + w * ph.rho_nt * ph.f_rec;
+            A = ak * A;
         }
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
@@ -179,173 +311,198 @@ __device__ void plasma_line(const float (&I)[EPT], const float (&wi)[EPT], float
         double rho = __shfl_up(Bc, 1, 64);
         if (lane == 0) rho = 0.0;
 #pragma unroll
-        for (int i = 0; i < EPT; ++i) {
-            const double w = (double)wi[i];
-            double depl = (ph.rho_nt - rho) / ph.rho_nt;
-            if (depl < 0.0) depl = 0.0;
-            const double gen = (w * ph.rho_nt) * depl;
-            const double cl = gen * ph.Ui / (2.0 * (double)I[i] + 1e-30);
-            const double cd = ph.half_sigma * rho;
-            c1[i] = (float)(cl + cd);
-            c2[i] = (float)(cd * ph.w0tc);
-            rho = rho * ph.e_rec + (w * ph.rho_nt * depl) * ph.f_rec;
-            rho = rho > ph.rho_nt ? ph.rho_nt : (rho < 0.0 ? 0.0 : rho);
+        for (int k = 0; k < R; ++k) {
+            float c1, c2;
+            k1_affine(rho, (double)wi[k], (double)(ph.a_scale2 * cabs2(x[k])), ph, c1, c2);
+            sink(k, c1, c2);
         }
         return;
     }
-    // serial path (reference order of operations, one lane)
+    double carry = 0.0;
+#pragma unroll 1
+    for (int L = 0; L < 64; ++L) {
+        double mine = 0.0;
+        if (lane == L) {
+            double rho = carry;
+#pragma unroll 1
+            for (int k = 0; k < R; ++k) {
+                const float I = ph.a_scale2 * cabs2(pick<R>(x, k));
+                const double wav = aval ? (double)(float)(ph.av_coef * (double)I) : 0.0;
+                float c1, c2;
+                k1_general(rho, (double)pick<R>(wi, k), wav, (double)I, ph, c1, c2);
 #pragma unroll
-    for (int i = 0; i < EPT; ++i) line[pidx(lane * EPT + i)] = make_float2(wi[i], I[i]);
-    wave_lds_sync();
-    if (lane == 0) {
-        double rho = 0.0;
-        for (int t = 0; t < NT; ++t) {
-            const float2 p = line[pidx(t)];
-            const double w = (double)p.x;
-            const double wav = aval ? (double)(float)(ph.av_coef * (double)p.y) : 0.0;
-            double depl = (ph.rho_nt - rho) / ph.rho_nt;
-            if (depl < 0.0) depl = 0.0;
-            const double gen = (w * ph.rho_nt + wav * rho) * depl;
-            const double S = w * ph.rho_nt * depl;
-            const double G = wav * depl - ph.r_rec;
-            double gdt = G * ph.dt;
-            gdt = gdt > 10.0 ? 10.0 : (gdt < -10.0 ? -10.0 : gdt);
-            const double eg = exp(gdt);
-            const double sf = fabs(gdt) < 1e-5 ? ph.dt * (1.0 + 0.5 * gdt) : (eg - 1.0) / G;
-            const double cl = gen * ph.Ui / (2.0 * (double)p.y + 1e-30);
-            const double cd = ph.half_sigma * rho;
-            line[pidx(t)] = make_float2((float)(cl + cd), (float)(cd * ph.w0tc));
-            rho = rho * eg + S * sf;
-            rho = rho > ph.rho_nt ? ph.rho_nt : (rho < 0.0 ? 0.0 : rho);
+                for (int kk = 0; kk < R; ++kk)
+                    if (kk == k) sink(kk, c1, c2);
+            }
+            mine = rho;
         }
+        carry = __shfl(mine, L, 64);
     }
-    wave_lds_sync();
+}
+
+// Whether the plasma term of a line can change the result at FP32 resolution.  With
+// I_max the line's peak intensity and W the (monotone) ionisation rate, every sample obeys
+// W_t <= W(I_max), W_t / I_t <= W(I_max) / I_max (W grows at least as fast as I), and
+// rho_t <= rho_nt min(1, f NT W(I_max)) (each recurrence step adds at most w f rho_nt).
+// Hence |c1|, |c2| are bounded by
+//   Ui rho_nt W(I_max) / (2 I_max) + sigma/2 rho_max (1 + omega0 tau_c).
+// When h times that bound is below 2^-40 the plasma contribution to every stage field is
+// far below one FP32 ulp and the line is treated as plasma-free.  A line whose intensities
+// all sit below the LUT's zero region has W = 0 exactly.
+template <int R>
+__device__ __forceinline__ bool plasma_matters(const float2 (&x)[R], const Phys3D &ph,
+                                               const double *__restrict__ lut, double h) {
+    float imax = 0.0f;
 #pragma unroll
-    for (int i = 0; i < EPT; ++i) {
-        const float2 p = line[pidx(lane * EPT + i)];
-        c1[i] = p.x;
-        c2[i] = p.y;
-    }
-    wave_lds_sync();
+    for (int k = 0; k < R; ++k) imax = fmaxf(imax, ph.a_scale2 * cabs2(x[k]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) imax = fmaxf(imax, __shfl_xor(imax, o, 64));
+    if (!((double)imax > ph.lut_zero_below)) return false;
+    if (ph.av_coef != 0.0 || !ph.lut_bound_ok) return true;
+    const double wmax = (double)lut_rate_inl(imax, lut, ph);
+    const double rho_max = ph.rho_nt * fmin(1.0, ph.f_rec * (64.0 * R) * wmax);
+    const double cbound = ph.Ui * ph.rho_nt * wmax / (2.0 * (double)imax) +
+                          ph.half_sigma * rho_max * (1.0 + ph.w0tc);
+    return !(h * cbound < 9.094947017729282e-13);
 }
 
 // ---------------------------------------------------------------------------------------
-// t-pass: Y = mask (shock FFT_t(Kerr) + FFT_t(Plasma)) of x = iFFT_t(X), LINES lines per
-// workgroup (fixed x, y0..y0+LINES-1), one wavefront per line.
-// MODE 0: out = Y.  MODE 1 (final stage): out = Y4 - Y, and during the load the next state
-// B' = X * filter(omega) * exp(-damp(x,y) h) is written to the spare state buffer.
-template <int NT, int LINES, int MODE>
+// t-pass: Y = mask (shock FFT_t(Kerr) + FFT_t(Plasma)) of x = iFFT_t(X).  A workgroup takes
+// LINES lines (fixed x, y0..y0+LINES-1), one wavefront per line; HBM is read and written in
+// rows of LINES*8 contiguous bytes and transposed through a small LDS stage, CH chunks of 64
+// omega rows at a time.
+// MODE 0: out = Y.  MODE 1 (final stage): out = Y4 - Y, and the next state
+// B' = X * filter(omega) * exp(-damp(x,y) h) is written to the spare state buffer on the way.
+//
+// Two instantiations per (NT, MODE): PLASMA = false has no plasma code (so the FFTs keep their
+// registers); a workgroup whose lines need the plasma term records that in flags[] and
+// returns before storing.  PLASMA = true then runs only the flagged workgroups.
+template <int NT, int LINES, int MODE, bool PLASMA>
 __global__ __launch_bounds__(64 * LINES) void tpass_kernel(Fields3D f, const float2 *in_fixed,
                                                            int in_is_state, float2 *out,
                                                            Tabs3D tb, Phys3D ph, Ctl3D *ctl,
-                                                           int nx, int ny) {
-    constexpr int EPT = NT / 64;
-    constexpr int LDT = NT + NT / 16;
-    constexpr int T = 64 * LINES;
-    __shared__ float2 lds[LINES * LDT];
+                                                           int nx, int ny, int *flags) {
+    constexpr int R = NT / 64;
+    constexpr int CH = R < 4 ? R : 4;
+    constexpr int SP = 66;   // padded stage row (float2): q-stride of 33 dword pairs
+    __shared__ float2 stage[CH * LINES * SP];
     if (ctl->done) return;
+    if (PLASMA && flags[blockIdx.x] == 0) return;
     const int cur = ctl->cur;
     const float2 *in = in_is_state ? (cur ? f.B1 : f.B0) : in_fixed;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int rl = __brev(lane) >> 26;
     const int bpx = ny / LINES;
     const int x = blockIdx.x / bpx;
     const int y0 = (blockIdx.x % bpx) * LINES;
     const size_t plane = (size_t)nx * ny;
     const size_t col0 = (size_t)x * ny + y0;
-    const int q = tid % LINES;
+    const int q = tid % LINES, row = tid / LINES;
+
+    float2 v[R];
     {
+        // HBM -> registers in chunks of CH rows-of-64, next chunk prefetched during the
+        // LDS transposition of the current one
         float absorb = 1.0f;
         float2 *bnew = nullptr;
         if (MODE == 1) {
             absorb = expf(-tb.damp[col0 + q] * (float)ctl->h);
             bnew = cur ? f.B0 : f.B1;
         }
-#pragma unroll 8
-        for (int i = 0; i < NT * LINES / T; ++i) {
-            const int w = tid / LINES + (T / LINES) * i;
-            const size_t g = (size_t)w * plane + col0 + q;
-            const float2 v = in[g];
-            lds[q * LDT + pidx(w)] = v;
-            if (MODE == 1) bnew[g] = cscale(v, tb.filt_w[w] * absorb);
-        }
-    }
-    __syncthreads();
-    const int wv = tid >> 6, lane = tid & 63;
-    float2 *line = lds + wv * LDT;
-    LineFFT<NT, EPT, 1, -1>::run(line, lane, tb.tw_t);
-    float2 a[EPT];
-    float I[EPT], wi[EPT], c1[EPT], c2[EPT];
-    const float invN = 1.0f / NT;
-    bool any = false;
+        float2 g[CH], gn[CH];
 #pragma unroll
-    for (int i = 0; i < EPT; ++i) {
-        a[i] = cscale(line[pidx(lane * EPT + i)], invN);
-        I[i] = ph.a_scale2 * cabs2(a[i]);
-        float w = 0.0f;
-        if ((double)I[i] > ph.lut_zero_below) w = lut_rate(I[i], tb.lut, ph);
-        wi[i] = w;
-        any |= (w > 0.0f);
-        c1[i] = 0.0f;
-        c2[i] = 0.0f;
-    }
-    const bool plasma = __ballot(any) != 0;
-    if (plasma) plasma_line<NT, EPT>(I, wi, c1, c2, line, lane, ph);
-    if (ph.use_shock) {
-        // Kerr alone, shock applied in omega; plasma (if any) in a second transform
+        for (int cc = 0; cc < CH; ++cc) g[cc] = in[(size_t)(row + 64 * cc) * plane + col0 + q];
 #pragma unroll
-        for (int i = 0; i < EPT; ++i) {
-            const float kap = I[i] * (ph.g_n2 + ph.g_n4 * I[i]);
-            line[pidx(lane * EPT + i)] = make_float2(-kap * a[i].y, kap * a[i].x);
-        }
-        wave_lds_sync();
-        LineFFT<NT, EPT, 1, 1>::run(line, lane, tb.tw_t);
-        float2 acc[EPT];
+        for (int c0 = 0; c0 < R; c0 += CH) {
+            if (c0 + CH < R) {
 #pragma unroll
-        for (int i = 0; i < EPT; ++i) {
-            const int w = lane + 64 * i;
-            acc[i] = cscale(line[pidx(w)], tb.shock_w[w] * tb.mask_w[w]);
-        }
-        if (plasma) {
-            wave_lds_sync();
-#pragma unroll
-            for (int i = 0; i < EPT; ++i)
-                line[pidx(lane * EPT + i)] =
-                    make_float2(-c1[i] * a[i].x + c2[i] * a[i].y, -c1[i] * a[i].y - c2[i] * a[i].x);
-            wave_lds_sync();
-            LineFFT<NT, EPT, 1, 1>::run(line, lane, tb.tw_t);
-#pragma unroll
-            for (int i = 0; i < EPT; ++i) {
-                const int w = lane + 64 * i;
-                acc[i] = caxpy(acc[i], tb.mask_w[w], line[pidx(w)]);
+                for (int cc = 0; cc < CH; ++cc)
+                    gn[cc] = in[(size_t)(row + 64 * (c0 + CH + cc)) * plane + col0 + q];
             }
-        }
-        wave_lds_sync();
+            if (MODE == 1) {
 #pragma unroll
-        for (int i = 0; i < EPT; ++i) line[pidx(lane + 64 * i)] = acc[i];
-    } else {
+                for (int cc = 0; cc < CH; ++cc) {
+                    const int w = row + 64 * (c0 + cc);
+                    bnew[(size_t)w * plane + col0 + q] = cscale(g[cc], tb.filt_w[w] * absorb);
+                }
+            }
 #pragma unroll
-        for (int i = 0; i < EPT; ++i) {
-            const float kap = I[i] * (ph.g_n2 + ph.g_n4 * I[i]);
-            const float im = kap - c2[i];
-            line[pidx(lane * EPT + i)] =
-                make_float2(-c1[i] * a[i].x - im * a[i].y, -c1[i] * a[i].y + im * a[i].x);
-        }
-        wave_lds_sync();
-        LineFFT<NT, EPT, 1, 1>::run(line, lane, tb.tw_t);
+            for (int cc = 0; cc < CH; ++cc) stage[(cc * LINES + q) * SP + row] = g[cc];
+            __syncthreads();
 #pragma unroll
-        for (int i = 0; i < EPT; ++i) {
-            const int w = lane + 64 * i;
-            line[pidx(w)] = cscale(line[pidx(w)], tb.mask_w[w]);
+            for (int cc = 0; cc < CH; ++cc) v[c0 + cc] = stage[(cc * LINES + wv) * SP + rl];
+            __syncthreads();
+#pragma unroll
+            for (int cc = 0; cc < CH; ++cc) g[cc] = gn[cc];
         }
     }
-    __syncthreads();
-#pragma unroll 8
-    for (int i = 0; i < NT * LINES / T; ++i) {
-        const int w = tid / LINES + (T / LINES) * i;
-        const size_t g = (size_t)w * plane + col0 + q;
-        float2 v = lds[q * LDT + pidx(w)];
-        if (MODE == 1) v = csub(f.Y4[g], v);
-        out[g] = v;
+    wave_ifft_t<R>(v, lane, rl, tb.tw_t);
+
+    bool plasma = plasma_matters<R>(v, ph, tb.lut, (double)ctl->h);
+    if (!PLASMA) {
+        const int defer = __syncthreads_or(plasma ? 1 : 0);
+        if (tid == 0) flags[blockIdx.x] = defer;
+        if (defer) return;
+        plasma = false;
+    }
+
+    if (!ph.use_shock) {
+        // N = mask FFT_t(Kerr + Plasma), formed in place
+        auto total = [&](int k, float c1, float c2) {
+            const float I = ph.a_scale2 * cabs2(v[k]);
+            const float im = I * (ph.g_n2 + ph.g_n4 * I) - c2;
+            v[k] = make_float2(-c1 * v[k].x - im * v[k].y, -c1 * v[k].y + im * v[k].x);
+        };
+        if (PLASMA && plasma) {
+            plasma_walk<R>(v, lane, ph, tb.lut, total);
+        } else {
+#pragma unroll
+            for (int k = 0; k < R; ++k) total(k, 0.0f, 0.0f);
+        }
+        wave_fft_t<R>(v, lane, rl, tb.tw_t);
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = cscale(v[r], tb.mask_w[rl + 64 * r]);
+    } else {
+        // N = mask (shock FFT_t(Kerr) + FFT_t(Plasma))
+        float2 p[R];
+        if (PLASMA && plasma) {
+            auto split = [&](int k, float c1, float c2) {
+                p[k] = make_float2(-c1 * v[k].x + c2 * v[k].y, -c1 * v[k].y - c2 * v[k].x);
+            };
+            plasma_walk<R>(v, lane, ph, tb.lut, split);
+        }
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const float I = ph.a_scale2 * cabs2(v[k]);
+            const float kap = I * (ph.g_n2 + ph.g_n4 * I);
+            v[k] = make_float2(-kap * v[k].y, kap * v[k].x);
+        }
+        wave_fft_t<R>(v, lane, rl, tb.tw_t);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int w = rl + 64 * r;
+            v[r] = cscale(v[r], tb.shock_w[w] * tb.mask_w[w]);
+        }
+        if (PLASMA && plasma) {
+            wave_fft_t<R>(p, lane, rl, tb.tw_t);
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[r] = caxpy(v[r], tb.mask_w[rl + 64 * r], p[r]);
+        }
+    }
+
+#pragma unroll
+    for (int c0 = 0; c0 < R; c0 += CH) {
+#pragma unroll
+        for (int cc = 0; cc < CH; ++cc) stage[(cc * LINES + wv) * SP + rl] = v[c0 + cc];
+        __syncthreads();
+#pragma unroll
+        for (int cc = 0; cc < CH; ++cc) {
+            const size_t g = (size_t)(row + 64 * (c0 + cc)) * plane + col0 + q;
+            float2 val = stage[(cc * LINES + q) * SP + row];
+            if (MODE == 1) val = csub(f.Y4[g], val);
+            out[g] = val;
+        }
+        __syncthreads();
     }
 }
 
@@ -748,6 +905,7 @@ struct uppe3d_plan {
     Tabs3D tb;
     Phys3D ph;
     Ctl3D *ctl;
+    int *flags;   // per t-pass workgroup: its lines need the plasma term
     size_t ws_bytes;
 };
 
@@ -758,7 +916,7 @@ size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 struct Layout3D {
     size_t tw_t, tw_x, tw_y, d_w, k_w, k2_w, kref_w, valid_w, filt_w, shock_w, mask_w, kx2, ky2,
-        damp, lut, ctl, fields[7], total;
+        damp, lut, ctl, flags, fields[7], total;
 };
 
 Layout3D layout3d(const uppe3d_desc &d) {
@@ -785,6 +943,7 @@ Layout3D layout3d(const uppe3d_desc &d) {
     L.damp = take(4 * (size_t)d.nx * d.ny);
     L.lut = take(8 * (size_t)d.lut_n);
     L.ctl = take(sizeof(Ctl3D));
+    L.flags = take(sizeof(int) * (size_t)d.nx * d.ny);
     const size_t field = sizeof(float2) * (size_t)d.nx * d.ny * d.nt;
     for (int i = 0; i < 7; ++i) L.fields[i] = take(field);
     L.total = o;
@@ -813,16 +972,19 @@ void twiddles(std::vector<float2> &v, int n) {
     }
 }
 
-#define LAUNCH_T(NT, LINES)                                                                    \
-    case NT:                                                                                   \
-        if (mode == 0)                                                                         \
-            tpass_kernel<NT, LINES, 0><<<grid, 64 * LINES, 0, s>>>(p->f, in, in_is_state, out, \
-                                                                  p->tb, p->ph, p->ctl, nx,    \
-                                                                  ny);                         \
-        else                                                                                   \
-            tpass_kernel<NT, LINES, 1><<<grid, 64 * LINES, 0, s>>>(p->f, in, in_is_state, out, \
-                                                                  p->tb, p->ph, p->ctl, nx,    \
-                                                                  ny);                         \
+#define LAUNCH_T(NT, LINES)                                                                  \
+    case NT:                                                                                 \
+        if (mode == 0) {                                                                     \
+            tpass_kernel<NT, LINES, 0, false><<<grid, 64 * LINES, 0, s>>>(                   \
+                p->f, in, in_is_state, out, p->tb, p->ph, p->ctl, nx, ny, p->flags);         \
+            tpass_kernel<NT, LINES, 0, true><<<grid, 64 * LINES, 0, s>>>(                    \
+                p->f, in, in_is_state, out, p->tb, p->ph, p->ctl, nx, ny, p->flags);         \
+        } else {                                                                             \
+            tpass_kernel<NT, LINES, 1, false><<<grid, 64 * LINES, 0, s>>>(                   \
+                p->f, in, in_is_state, out, p->tb, p->ph, p->ctl, nx, ny, p->flags);         \
+            tpass_kernel<NT, LINES, 1, true><<<grid, 64 * LINES, 0, s>>>(                    \
+                p->f, in, in_is_state, out, p->tb, p->ph, p->ctl, nx, ny, p->flags);         \
+        }                                                                                    \
         break;
 
 int launch_tpass(uppe3d_plan *p, int mode, const float2 *in, int in_is_state, float2 *out,
@@ -981,6 +1143,7 @@ int uppe3d_plan_create(const uppe3d_desc *d, void *workspace, size_t workspace_b
     p->d = *d;
     p->ws_bytes = L.total;
     p->ctl = (Ctl3D *)(ws + L.ctl);
+    p->flags = (int *)(ws + L.flags);
     float2 **fp[7] = {&p->f.B0, &p->f.B1, &p->f.X, &p->f.Yw, &p->f.Y4, &p->f.V, &p->f.Q};
     for (int i = 0; i < 7; ++i) *fp[i] = (float2 *)(ws + L.fields[i]);
     Tabs3D &tb = p->tb;
@@ -1033,6 +1196,15 @@ int uppe3d_plan_create(const uppe3d_desc *d, void *workspace, size_t workspace_b
         ph.f_rec = fabs(gdt) < 1e-5 ? ph.dt * (1.0 + 0.5 * gdt) : (ph.e_rec - 1.0) / G;
     }
     ph.use_shock = d->use_shock ? 1 : 0;
+    // the per-line plasma bound needs W(I) and W(I)/I nondecreasing wherever W != 0 in float32,
+    // and W = 0 below the grid (the clip region)
+    ph.lut_bound_ok = j0 >= 2 ? 1 : 0;
+    for (int j = j0; j + 1 < d->lut_n && ph.lut_bound_ok; ++j) {
+        const double I0 = pow(10.0, ph.lut_x0 + j * ph.lut_dx);
+        const double I1 = pow(10.0, ph.lut_x0 + (j + 1) * ph.lut_dx);
+        if (!(lut_w[j + 1] >= lut_w[j]) || !(lut_w[j + 1] / I1 >= lut_w[j] / I0 * (1.0 - 1e-12)))
+            ph.lut_bound_ok = 0;
+    }
     (void)s;
     *out = p;
     return UPPE_OK;

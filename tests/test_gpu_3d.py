@@ -35,9 +35,9 @@ CASES = [
     ("16x128 kerr+hoke", 16, 128, 0.8e-3, True, False, 0.02, 1e-5, 2e-5),
     ("32x256 kerr", 32, 256, 0.8e-3, False, False, 0.5, 1e-5, 2e-5),
     ("32x256 shock+hoke", 32, 256, 0.8e-3, True, True, 0.01, 1e-5, 2e-5),
-    # narrow window (half width 1 mm, w0 = 0.3 mm): peak 3e17 W/m^2 exercises the FP64 affine
+    # narrow window (half width 1 mm, w0 = 0.3 mm): peak 6e17 W/m^2 exercises the FP64 affine
     # scan; peak 5e18 W/m^2 makes W dt > exp(gdt) and takes the serial recurrence
-    ("32x256 plasma scan", 32, 256, 1.8e-3, False, False, 0.005, 1e-5, 2e-5),
+    ("32x256 plasma scan", 32, 256, 4e-3, False, False, 0.005, 1e-5, 2e-5),
     ("32x256 plasma stress", 32, 256, 30e-3, False, True, 0.001, 1e-4, 1e-3),
 ]
 
