@@ -1,0 +1,736 @@
+// 1D envelope UPPE / GNLSE, ERK4(3) in the interaction picture, on gfx950.
+//
+// Replaces ERK43IP_UPPE.propagate / erk43_step / linear_operator / nonlinear_operator
+// (src/1D-ERK4(3)IP UPPE gas/erk43ip_method.py:181-369; argon patch gas.py:11-44 acts on the
+// host-side tables).  One workgroup owns one pulse instance for the whole propagation: the
+// adaptive loop, controller, error norm and history rule all run on the device, so a single
+// launch serves one pulse or a batch of pulses of uneven length (persistent kernel, each
+// workgroup takes instances from a device counter).
+//
+// State: A(omega) (FFT order), so the interaction-picture propagator exp(h/2 D) is diagonal.
+// One attempt (FSAL: N(A) is kept from the previous attempt while A is unchanged):
+//   v = E A, a1 = E N(A), a2 = N(v + h/2 a1), a3 = N(v + h/2 a2), a4 = N(E (v + h a3)),
+//   beta = E (v + h/6 (a1 + 2 a2 + 2 a3)), A4 = beta + h/6 a4, a5 = N(A4),
+//   err = ||A4 - A3|| / ||A4|| with A4 - A3 = (h/10)(a4 - a5), norms in FP64 (time-domain
+//   values by Parseval).  N(F) = FFT(iγ·uppe·FFT... ) as in erk43ip_method.py:197-255 plus the
+//   optional plasma term (1D restatement of media_physics.py:156-205).
+// Fields live in registers (element e = tid + T*i), FFTs go through two LDS buffers.
+#include <math.h>
+#include <string.h>
+
+#include <vector>
+
+#include "abi_common.hpp"
+#include "fft_common.hpp"
+
+using namespace uppe;
+
+namespace {
+
+constexpr int MAXE = 8;   // elements per thread
+
+struct Stat1D {
+    double z, h, err;
+    int attempts, accepted, rejected, forced, nhist, done, nonfinite, fsal_hits;
+    int hist_overflow, pad_;
+};
+
+struct Inst1D {
+    int nt, max_hist, max_attempts, use_ss;
+    int use_raman, use_plasma, K, avalanche;
+    double gamma, f_R, L, tol, max_step, min_step;
+    double sigma_K, inv_area, Ui, rho_nt, r_rec, dt, half_sigma, w0tc, av_coef, e_rec, f_rec;
+    const double *D_re, *D_im;
+    const float *uppe;
+    const float2 *H, *tw;
+    const float2 *a0;      // input, time domain
+    float2 *A, *a5, *v;    // state (omega), FSAL term, IP-transformed state
+    float2 *hist;          // [max_hist][nt], time domain
+    double *zhist;         // [max_hist]
+    Stat1D *st;
+};
+
+// Stockham stage of radix R over LDS: src -> dst.
+template <int T, int R, int DIR>
+__device__ __forceinline__ void lds_stage(const float2 *src, float2 *dst, int n, int ns,
+                                          const float2 *__restrict__ tw, int tid) {
+    const int nb = n / R;
+    for (int j = tid; j < nb; j += T) {
+        float2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = src[j + r * nb];
+        const int b = stockham_butterfly<R, DIR>(v, j, ns, n, tw);
+#pragma unroll
+        for (int r = 0; r < R; ++r) dst[b + r * ns] = v[r];
+    }
+}
+
+// Length-n FFT of the data in a (natural order); returns the buffer holding the result.
+template <int T, int DIR>
+__device__ float2 *lds_fft(float2 *a, float2 *b, int n, const float2 *__restrict__ tw, int tid) {
+    int ns = 1;
+    float2 *src = a, *dst = b;
+    while (ns < n) {
+        const int rem = n / ns;
+        if (rem >= 8) {
+            lds_stage<T, 8, DIR>(src, dst, n, ns, tw, tid);
+            ns *= 8;
+        } else if (rem == 4) {
+            lds_stage<T, 4, DIR>(src, dst, n, ns, tw, tid);
+            ns *= 4;
+        } else {
+            lds_stage<T, 2, DIR>(src, dst, n, ns, tw, tid);
+            ns *= 2;
+        }
+        __syncthreads();
+        float2 *t = src;
+        src = dst;
+        dst = t;
+    }
+    return src;
+}
+
+template <int T>
+__device__ double block_sum(double v, double *red, int tid) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    __syncthreads();
+    if ((tid & 63) == 0) red[tid >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < T / 64; ++w) s += red[w];
+    return s;
+}
+
+// One reference recurrence step (media_physics.py:36-63) with general avalanche rate.
+__device__ __noinline__ void k1_step(double &rho, double w, double wav, double I,
+                                     const Inst1D &P, float &c1, float &c2) {
+    double depl = (P.rho_nt - rho) / P.rho_nt;
+    if (depl < 0.0) depl = 0.0;
+    const double gen = (w * P.rho_nt + wav * rho) * depl;
+    const double S = w * P.rho_nt * depl;
+    double e = P.e_rec, f = P.f_rec;
+    if (wav != 0.0) {
+        const double G = wav * depl - P.r_rec;
+        double gdt = G * P.dt;
+        gdt = gdt > 10.0 ? 10.0 : (gdt < -10.0 ? -10.0 : gdt);
+        e = exp(gdt);
+        f = fabs(gdt) < 1e-5 ? P.dt * (1.0 + 0.5 * gdt) : (e - 1.0) / G;
+    }
+    const double cl = gen * P.Ui / (2.0 * I + 1e-30);
+    const double cd = P.half_sigma * rho;
+    c1 = (float)(cl + cd);
+    c2 = (float)(cd * P.w0tc);
+    rho = rho * e + S * f;
+    rho = rho > P.rho_nt ? P.rho_nt : (rho < 0.0 ? 0.0 : rho);
+}
+
+// Plasma coefficients of the line: on entry buf[t] = (W_ion(t), I(t)); on exit
+// buf[t] = (c1, c2) with plasma term -(c1 + i c2) A(t).  Thread tid owns the contiguous
+// samples [tid*E, tid*E + E).  Affine prefix scan in FP64 when the avalanche rate is zero
+// and no clamp can act; otherwise thread 0 runs the serial recurrence.
+template <int T>
+__device__ void plasma_coeffs_1d(float2 *buf, int n, const Inst1D &P, double *scan, int tid) {
+    const int E = n / T;
+    const int t0 = tid * E;
+    bool risk = false;
+    for (int k = 0; k < E; ++k) risk |= ((double)buf[t0 + k].x * P.f_rec > P.e_rec);
+    const bool serial = P.avalanche || __syncthreads_or(risk ? 1 : 0);
+    if (!serial) {
+        double A = 1.0, B = 0.0;
+        for (int k = 0; k < E; ++k) {
+            const double w = (double)buf[t0 + k].x;
+            const double ak = P.e_rec - w * P.f_rec;
+            B = ak * B + w * P.rho_nt * P.f_rec;
+            A = ak * A;
+        }
+        const int lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const double Ao = __shfl_up(A, off, 64), Bo = __shfl_up(B, off, 64);
+            if (lane >= off) {
+                B = A * Bo + B;
+                A = A * Ao;
+            }
+        }
+        double Ae = __shfl_up(A, 1, 64), Be = __shfl_up(B, 1, 64);
+        if (lane == 0) {
+            Ae = 1.0;
+            Be = 0.0;
+        }
+        if (lane == 63) {
+            scan[2 * wv] = A;
+            scan[2 * wv + 1] = B;
+        }
+        __syncthreads();
+        if (tid == 0) {   // exclusive prefix over the waves, in place
+            double pa = 1.0, pb = 0.0;
+            for (int w = 0; w < T / 64; ++w) {
+                const double a = scan[2 * w], b = scan[2 * w + 1];
+                scan[2 * w] = pa;
+                scan[2 * w + 1] = pb;
+                pb = a * pb + b;
+                pa = a * pa;
+            }
+        }
+        __syncthreads();
+        double rho = Ae * scan[2 * wv + 1] + Be;
+        for (int k = 0; k < E; ++k) {
+            const float2 p = buf[t0 + k];
+            float c1, c2;
+            k1_step(rho, (double)p.x, 0.0, (double)p.y, P, c1, c2);
+            buf[t0 + k] = make_float2(c1, c2);
+        }
+        __syncthreads();
+        return;
+    }
+    if (tid == 0) {
+        double rho = 0.0;
+        for (int t = 0; t < n; ++t) {
+            const float2 p = buf[t];
+            const double wav =
+                P.avalanche ? (double)(float)(P.av_coef * (double)p.y) : 0.0;
+            float c1, c2;
+            k1_step(rho, (double)p.x, wav, (double)p.y, P, c1, c2);
+            buf[t] = make_float2(c1, c2);
+        }
+    }
+    __syncthreads();
+}
+
+// N(F): F in omega (registers, e = tid + T i) -> out in omega.  If snap != nullptr the time
+// samples of F are also written there (history slot).
+template <int T, int NMAX>
+__device__ void neval(const Inst1D &P, const float2 (&F)[MAXE], float2 (&out)[MAXE], float2 *bufA,
+                      float2 *bufB, double *scan, int tid, float2 *snap) {
+    const int n = P.nt, E = n / T;
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i)
+        if (i < E) bufA[tid + T * i] = F[i];
+    __syncthreads();
+    float2 *r = lds_fft<T, -1>(bufA, bufB, n, P.tw, tid);
+    const float invn = 1.0f / (float)n;
+    float2 a[MAXE], Ie[MAXE];
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i)
+        if (i < E) {
+            a[i] = cscale(r[tid + T * i], invn);
+            Ie[i] = make_float2(cabs2(a[i]), 0.0f);
+            if (snap) snap[tid + T * i] = a[i];
+        }
+    __syncthreads();
+    if (P.use_raman) {
+        // I_eff = (1 - f_R) I + f_R ifft(fft(I) H)   (complex, erk43ip_method.py:206-217)
+#pragma unroll
+        for (int i = 0; i < MAXE; ++i)
+            if (i < E) bufA[tid + T * i] = Ie[i];
+        __syncthreads();
+        float2 *q = lds_fft<T, 1>(bufA, bufB, n, P.tw, tid);
+        float2 *o = (q == bufA) ? bufB : bufA;
+#pragma unroll
+        for (int i = 0; i < MAXE; ++i)
+            if (i < E) o[tid + T * i] = cmul(q[tid + T * i], P.H[tid + T * i]);
+        __syncthreads();
+        float2 *o2 = (o == bufA) ? bufB : bufA;
+        float2 *d = lds_fft<T, -1>(o, o2, n, P.tw, tid);
+        const float fr = (float)P.f_R;
+#pragma unroll
+        for (int i = 0; i < MAXE; ++i)
+            if (i < E) {
+                const float2 idl = cscale(d[tid + T * i], invn);
+                Ie[i] = make_float2((1.0f - fr) * Ie[i].x + fr * idl.x, fr * idl.y);
+            }
+        __syncthreads();
+    }
+    // plasma coefficients (zero unless the model is on and some rate is non-zero)
+    float c1[MAXE], c2[MAXE];
+    bool pl = false;
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i) {
+        c1[i] = 0.0f;
+        c2[i] = 0.0f;
+    }
+    if (P.use_plasma) {
+        bool any = false;
+#pragma unroll
+        for (int i = 0; i < MAXE; ++i)
+            if (i < E) {
+                const double I = (double)cabs2(a[i]) * P.inv_area;
+                double pw = 1.0;   // I^K by repeated multiplication (integer K)
+                for (int k = 0; k < P.K; ++k) pw *= I;
+                const float w = (float)(P.sigma_K * pw);
+                bufA[tid + T * i] = make_float2(w, (float)I);
+                any |= (w > 0.0f);
+            }
+        pl = __syncthreads_or(any ? 1 : 0) != 0;
+        if (pl) {
+            plasma_coeffs_1d<T>(bufA, n, P, scan, tid);
+#pragma unroll
+            for (int i = 0; i < MAXE; ++i)
+                if (i < E) {
+                    const float2 c = bufA[tid + T * i];
+                    c1[i] = c.x;
+                    c2[i] = c.y;
+                }
+        }
+        __syncthreads();
+    }
+    // P_NL = A I_eff; plasma term -(c1 + i c2) A
+    const float g = (float)P.gamma;
+    if (P.use_ss) {
+        // out = ifft-free omega form: iγ uppe(ω) FFT(P_NL) + FFT(plasma)
+#pragma unroll
+        for (int i = 0; i < MAXE; ++i)
+            if (i < E) bufA[tid + T * i] = cmul(a[i], Ie[i]);
+        __syncthreads();
+        float2 *k = lds_fft<T, 1>(bufA, bufB, n, P.tw, tid);
+#pragma unroll
+        for (int i = 0; i < MAXE; ++i)
+            if (i < E) {
+                const int e = tid + T * i;
+                const float s = g * P.uppe[e];
+                const float2 v = k[e];
+                out[i] = make_float2(-s * v.y, s * v.x);
+            }
+        __syncthreads();
+        if (pl) {
+#pragma unroll
+            for (int i = 0; i < MAXE; ++i)
+                if (i < E)
+                    bufA[tid + T * i] = make_float2(-c1[i] * a[i].x + c2[i] * a[i].y,
+                                                    -c1[i] * a[i].y - c2[i] * a[i].x);
+            __syncthreads();
+            float2 *q = lds_fft<T, 1>(bufA, bufB, n, P.tw, tid);
+#pragma unroll
+            for (int i = 0; i < MAXE; ++i)
+                if (i < E) out[i] = cadd(out[i], q[tid + T * i]);
+            __syncthreads();
+        }
+    } else {
+        // iγ P_NL + plasma in time, one FFT
+#pragma unroll
+        for (int i = 0; i < MAXE; ++i)
+            if (i < E) {
+                const float2 p = cmul(a[i], Ie[i]);
+                bufA[tid + T * i] = make_float2(-g * p.y - c1[i] * a[i].x + c2[i] * a[i].y,
+                                                g * p.x - c1[i] * a[i].y - c2[i] * a[i].x);
+            }
+        __syncthreads();
+        float2 *k = lds_fft<T, 1>(bufA, bufB, n, P.tw, tid);
+#pragma unroll
+        for (int i = 0; i < MAXE; ++i)
+            if (i < E) out[i] = k[tid + T * i];
+        __syncthreads();
+    }
+}
+
+// exp(s D) for the thread's elements; the phase s*D_im is formed and reduced in FP64.
+__device__ __forceinline__ float2 ipexp(const Inst1D &P, int e, double s) {
+    double sn, cs;
+    sincos(s * P.D_im[e], &sn, &cs);
+    const double m = exp(s * P.D_re[e]);
+    return make_float2((float)(m * cs), (float)(m * sn));
+}
+
+// One attempt of size h from state A (global) with FSAL term a5 (global).  Produces A4 in
+// A4o, N(A4) in a5o, returns err (all threads).  snap: history slot for A4's time samples.
+template <int T, int NMAX>
+__device__ double attempt(const Inst1D &P, double h, float2 (&A4o)[MAXE], float2 (&a5o)[MAXE],
+                          float2 *bufA, float2 *bufB, double *scan, int tid, float2 *snap) {
+    const int E = P.nt / T;
+    const float hf = (float)h;
+    float2 Ef[MAXE], v[MAXE], S[MAXE], x[MAXE], y[MAXE];
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i)
+        if (i < E) {
+            const int e = tid + T * i;
+            Ef[i] = ipexp(P, e, 0.5 * h);
+            v[i] = cmul(Ef[i], P.A[e]);
+            const float2 a1 = cmul(Ef[i], P.a5[e]);
+            S[i] = caxpy(v[i], hf / 6.0f, a1);
+            x[i] = caxpy(v[i], 0.5f * hf, a1);
+        }
+    neval<T, NMAX>(P, x, y, bufA, bufB, scan, tid, nullptr);   // a2
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i)
+        if (i < E) {
+            S[i] = caxpy(S[i], hf / 3.0f, y[i]);
+            x[i] = caxpy(v[i], 0.5f * hf, y[i]);
+        }
+    neval<T, NMAX>(P, x, y, bufA, bufB, scan, tid, nullptr);   // a3
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i)
+        if (i < E) {
+            S[i] = caxpy(S[i], hf / 3.0f, y[i]);
+            x[i] = cmul(Ef[i], caxpy(v[i], hf, y[i]));
+        }
+    neval<T, NMAX>(P, x, y, bufA, bufB, scan, tid, nullptr);   // a4
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i)
+        if (i < E) {
+            A4o[i] = caxpy(cmul(Ef[i], S[i]), hf / 6.0f, y[i]);   // beta + h/6 a4
+            x[i] = y[i];                                          // keep a4
+        }
+    neval<T, NMAX>(P, A4o, a5o, bufA, bufB, scan, tid, snap);  // a5
+    double sd = 0.0, sa = 0.0;
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i)
+        if (i < E) {
+            const float2 d = csub(x[i], a5o[i]);
+            sd += (double)d.x * d.x + (double)d.y * d.y;
+            sa += (double)A4o[i].x * A4o[i].x + (double)A4o[i].y * A4o[i].y;
+        }
+    sd = block_sum<T>(sd, scan, tid);
+    sa = block_sum<T>(sa, scan + 32, tid);
+    // time-domain norms (Parseval: sum_t |x|^2 = sum_w |X|^2 / n)
+    double na = sqrt(sa / (double)P.nt);
+    if (na < 1e-20) na = 1.0;
+    return (h / 10.0) * sqrt(sd / (double)P.nt) / na;
+}
+
+template <int T, int NMAX>
+__device__ void run_instance(const Inst1D &P, float2 *bufA, float2 *bufB, double *scan,
+                             double *bc, int tid) {
+    const int n = P.nt, E = n / T;
+    // A0 -> history slot 0 and state FFT(A0); a5 = N(A)
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i)
+        if (i < E) {
+            const int e = tid + T * i;
+            const float2 a0 = P.a0[e];
+            if (P.max_hist > 0) P.hist[e] = a0;
+            bufA[e] = a0;
+        }
+    __syncthreads();
+    {
+        float2 *r = lds_fft<T, 1>(bufA, bufB, n, P.tw, tid);
+        float2 A[MAXE], N0[MAXE];
+#pragma unroll
+        for (int i = 0; i < MAXE; ++i)
+            if (i < E) {
+                A[i] = r[tid + T * i];
+                P.A[tid + T * i] = A[i];
+            }
+        __syncthreads();
+        neval<T, NMAX>(P, A, N0, bufA, bufB, scan, tid, nullptr);
+#pragma unroll
+        for (int i = 0; i < MAXE; ++i)
+            if (i < E) P.a5[tid + T * i] = N0[i];
+    }
+    // controller state (thread 0 owns it; broadcast through bc[])
+    double z = 0.0, h = P.max_step / 100.0, eprev = -1.0;   // eprev < 0: None
+    int nh = P.max_hist > 0 ? 1 : 0, attempts = 0, acc = 0, rej = 0, forced = 0, nonfinite = 0;
+    int overflow = 0;
+    double zlast = 0.0;
+    if (nh > 0 && tid == 0) P.zhist[0] = 0.0;
+    double err = 0.0;
+    float2 A4[MAXE], a5[MAXE];
+    while (z < P.L) {
+        if (P.max_attempts > 0 && attempts >= P.max_attempts) break;
+        if (z + h > P.L) h = P.L - z;
+        const int slot = nh < P.max_hist ? nh : (P.max_hist > 0 ? P.max_hist - 1 : -1);
+        float2 *snap = slot >= 0 ? P.hist + (size_t)slot * n : nullptr;
+        err = attempt<T, NMAX>(P, h, A4, a5, bufA, bufB, scan, tid, snap);
+        ++attempts;
+        if (!(err == err) || isinf(err)) {
+            nonfinite = 1;
+            break;
+        }
+        bool accept = err < P.tol;
+        bool record = false;
+        if (accept) {
+            z += h;
+            double fac = 0.95 * pow(P.tol / (err + 1e-30), 0.25);
+            if (eprev >= 0.0 && eprev > 1e-30) fac *= pow(eprev / (err + 1e-30), 0.1);
+            eprev = err;
+            const double hn = h * (fac < 2.0 ? fac : 2.0);
+            h = hn < P.max_step ? hn : P.max_step;
+            ++acc;
+        } else {
+            ++rej;
+            const double fac = 0.95 * pow(P.tol / (err + 1e-30), 0.25);
+            const double hn = h * (fac > 0.1 ? fac : 0.1);
+            h = hn > P.min_step ? hn : P.min_step;
+            if (h <= P.min_step) {
+                // forced step at min_step (erk43ip_method.py:344-367)
+                err = attempt<T, NMAX>(P, h, A4, a5, bufA, bufB, scan, tid, snap);
+                ++attempts;
+                ++forced;
+                if (!(err == err) || isinf(err)) {
+                    nonfinite = 1;
+                    break;
+                }
+                z += h;
+                eprev = -1.0;
+                accept = true;
+            }
+        }
+        if (accept) {
+            // history rule of erk43ip_method.py:327-329
+            record = (nh < 1000) || (z - zlast > P.L / 500.0);
+#pragma unroll
+            for (int i = 0; i < MAXE; ++i)
+                if (i < E) {
+                    P.A[tid + T * i] = A4[i];
+                    P.a5[tid + T * i] = a5[i];
+                }
+            __syncthreads();
+            if (record && slot >= 0) {
+                if (tid == 0) P.zhist[slot] = z;
+                zlast = z;
+                if (nh < P.max_hist) ++nh;
+                else overflow = 1;
+            }
+            __syncthreads();
+        }
+    }
+    if (tid == 0) {
+        Stat1D s;
+        s.z = z;
+        s.h = h;
+        s.err = err;
+        s.attempts = attempts;
+        s.accepted = acc;
+        s.rejected = rej;
+        s.forced = forced;
+        s.nhist = nh;
+        s.done = (z >= P.L) ? 1 : 0;
+        s.nonfinite = nonfinite;
+        s.fsal_hits = attempts > 0 ? attempts - 0 : 0;
+        s.hist_overflow = overflow;
+        s.pad_ = 0;
+        *P.st = s;
+    }
+    __syncthreads();
+}
+
+__global__ void zero_counter(int *c) { *c = 0; }
+
+template <int T, int NMAX>
+__global__ __launch_bounds__(T) void uppe1d_kernel(const Inst1D *__restrict__ insts, int n_inst,
+                                                   int *counter) {
+    __shared__ float2 bufA[NMAX], bufB[NMAX];
+    __shared__ double scan[64];
+    __shared__ double bc[8];
+    __shared__ int next;
+    const int tid = threadIdx.x;
+    for (;;) {
+        if (tid == 0) next = atomicAdd(counter, 1);
+        __syncthreads();
+        const int ii = next;
+        __syncthreads();
+        if (ii >= n_inst) break;
+        const Inst1D P = insts[ii];
+        run_instance<T, NMAX>(P, bufA, bufB, scan, bc, tid);
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// Host side
+namespace {
+
+int check1d(const uppe1d_desc &d, int threads) {
+    UPPE_CHECK(d.nt >= 256 && d.nt <= 8192 && (d.nt & (d.nt - 1)) == 0,
+               "uppe1d: nt = %d unsupported (power of two in [256, 8192])", d.nt);
+    UPPE_CHECK(d.nt % threads == 0 && d.nt / threads <= MAXE && d.nt >= threads,
+               "uppe1d: nt = %d needs %d..%d threads (got %d)", d.nt, d.nt / MAXE, d.nt, threads);
+    UPPE_CHECK(d.tol > 0.0 && d.max_step > 0.0 && d.min_step >= 0.0 && d.L >= 0.0,
+               "uppe1d: bad tolerance / step arguments");
+    UPPE_CHECK(!d.use_plasma || (d.K >= 1 && d.area > 0.0 && d.rho_nt > 0.0 && d.tau_rec > 0.0),
+               "uppe1d: plasma needs K >= 1, area > 0, rho_nt > 0, tau_rec > 0");
+    return UPPE_OK;
+}
+
+size_t align256(size_t v) { return (v + 255) / 256 * 256; }
+
+// per-instance workspace: D_re, D_im (f64), uppe (f32), H, tw (c64), A, a5, v (c64), status
+size_t inst_bytes(int nt) {
+    return align256(16 * (size_t)nt) + align256(4 * (size_t)nt) + 5 * align256(8 * (size_t)nt) +
+           align256(sizeof(Stat1D));
+}
+
+int default_threads(int nt) { return nt <= 2048 ? 256 : nt / MAXE; }
+
+template <int T>
+int launch1d(const Inst1D *d_insts, int n, int *counter, hipStream_t s, int nt_max) {
+    const int grid = n;
+    if (nt_max <= 2048) uppe1d_kernel<T, 2048><<<grid, T, 0, s>>>(d_insts, n, counter);
+    else uppe1d_kernel<T, 8192><<<grid, T, 0, s>>>(d_insts, n, counter);
+    UPPE_HIP(hipGetLastError());
+    return UPPE_OK;
+}
+
+int run_batch(int n, const uppe1d_desc *descs, const double *D_re, const double *D_im,
+              const double *uppe_factor, const double *H_re, const double *H_im, const void *a0,
+              void *workspace, void *hist, void *zhist, int threads, void *stream,
+              uppe1d_status *st) {
+    UPPE_CHECK(n > 0 && descs && D_re && D_im && uppe_factor && a0 && workspace && st,
+               "uppe1d: null argument");
+    int nt_max = 0;
+    for (int i = 0; i < n; ++i) nt_max = descs[i].nt > nt_max ? descs[i].nt : nt_max;
+    if (threads <= 0) threads = default_threads(nt_max);
+    UPPE_CHECK(threads == 256 || threads == 512 || threads == 1024,
+               "uppe1d: threads must be 256, 512 or 1024");
+    for (int i = 0; i < n; ++i) {
+        int r = check1d(descs[i], threads);
+        if (r) return r;
+        UPPE_CHECK(descs[i].max_hist <= 0 || (hist && zhist), "uppe1d: history buffers missing");
+    }
+    hipStream_t s = (hipStream_t)stream;
+    char *ws = (char *)workspace;
+    std::vector<Inst1D> ins(n);
+    size_t off = align256(sizeof(Inst1D) * n) + 256;   // device copy of the descriptors + counter
+    size_t toff = 0, hoff = 0, zoff = 0;
+    const float2 *a0c = (const float2 *)a0;
+    for (int i = 0; i < n; ++i) {
+        const uppe1d_desc &d = descs[i];
+        const int nt = d.nt;
+        Inst1D &P = ins[i];
+        memset(&P, 0, sizeof P);
+        P.nt = nt;
+        P.max_hist = d.max_hist > 0 ? d.max_hist : 0;
+        P.max_attempts = d.max_attempts;
+        P.use_ss = d.use_ss;
+        P.use_raman = d.use_raman && H_re && H_im;
+        P.use_plasma = d.use_plasma;
+        P.K = d.K;
+        P.avalanche = d.avalanche;
+        P.gamma = d.gamma;
+        P.f_R = d.f_R;
+        P.L = d.L;
+        P.tol = d.tol;
+        P.max_step = d.max_step;
+        P.min_step = d.min_step;
+        if (d.use_plasma) {
+            P.sigma_K = d.sigma_K;
+            P.inv_area = 1.0 / d.area;
+            P.Ui = d.Ui;
+            // the K1 launch casts these to float32 (media_physics.py:185-188)
+            P.rho_nt = (double)(float)d.rho_nt;
+            P.r_rec = (double)(float)(1.0 / d.tau_rec);
+            P.dt = (double)(float)d.dt;
+            P.half_sigma = d.sigma_drude / 2.0;
+            P.w0tc = d.omega0 * d.tau_c;
+            P.av_coef = d.avalanche ? d.sigma_drude / d.Ui : 0.0;
+            double gdt = -P.r_rec * P.dt;
+            gdt = gdt > 10.0 ? 10.0 : (gdt < -10.0 ? -10.0 : gdt);
+            P.e_rec = exp(gdt);
+            P.f_rec = fabs(gdt) < 1e-5 ? P.dt * (1.0 + 0.5 * gdt) : (P.e_rec - 1.0) / (-P.r_rec);
+        }
+        // tables
+        char *base = ws + off;
+        size_t o = 0;
+        std::vector<double> dd(2 * (size_t)nt);
+        for (int k = 0; k < nt; ++k) {
+            dd[k] = D_re[toff + k];
+            dd[nt + k] = D_im[toff + k];
+        }
+        UPPE_HIP(hipMemcpyAsync(base + o, dd.data(), 16 * (size_t)nt, hipMemcpyHostToDevice, s));
+        P.D_re = (const double *)(base + o);
+        P.D_im = (const double *)(base + o) + nt;
+        o += align256(16 * (size_t)nt);
+        std::vector<float> uf(nt);
+        for (int k = 0; k < nt; ++k) uf[k] = (float)uppe_factor[toff + k];
+        UPPE_HIP(hipMemcpyAsync(base + o, uf.data(), 4 * (size_t)nt, hipMemcpyHostToDevice, s));
+        P.uppe = (const float *)(base + o);
+        o += align256(4 * (size_t)nt);
+        std::vector<float2> hv(nt), tv(nt);
+        for (int k = 0; k < nt; ++k) {
+            hv[k] = P.use_raman ? make_float2((float)H_re[toff + k], (float)H_im[toff + k])
+                                : make_float2(1.0f, 0.0f);
+            const double a = -2.0 * M_PI * (double)k / (double)nt;
+            tv[k] = make_float2((float)cos(a), (float)sin(a));
+        }
+        UPPE_HIP(hipMemcpyAsync(base + o, hv.data(), 8 * (size_t)nt, hipMemcpyHostToDevice, s));
+        P.H = (const float2 *)(base + o);
+        o += align256(8 * (size_t)nt);
+        UPPE_HIP(hipMemcpyAsync(base + o, tv.data(), 8 * (size_t)nt, hipMemcpyHostToDevice, s));
+        P.tw = (const float2 *)(base + o);
+        o += align256(8 * (size_t)nt);
+        P.A = (float2 *)(base + o);
+        o += align256(8 * (size_t)nt);
+        P.a5 = (float2 *)(base + o);
+        o += align256(8 * (size_t)nt);
+        P.v = (float2 *)(base + o);
+        o += align256(8 * (size_t)nt);
+        P.st = (Stat1D *)(base + o);
+        o += align256(sizeof(Stat1D));
+        off += o;
+        // the host vectors above must outlive the async copies
+        UPPE_HIP(hipStreamSynchronize(s));
+        P.a0 = a0c + toff;
+        P.hist = P.max_hist > 0 ? (float2 *)hist + hoff : nullptr;
+        P.zhist = P.max_hist > 0 ? (double *)zhist + zoff : nullptr;
+        toff += nt;
+        hoff += (size_t)P.max_hist * nt;
+        zoff += P.max_hist;
+    }
+    UPPE_HIP(hipMemcpyAsync(ws, ins.data(), sizeof(Inst1D) * n, hipMemcpyHostToDevice, s));
+    int *counter = (int *)(ws + align256(sizeof(Inst1D) * n));
+    zero_counter<<<1, 1, 0, s>>>(counter);   // a kernel, not a memset node (graph capture)
+    int r;
+    switch (threads) {
+        case 256: r = launch1d<256>((const Inst1D *)ws, n, counter, s, nt_max); break;
+        case 512: r = launch1d<512>((const Inst1D *)ws, n, counter, s, nt_max); break;
+        default: r = launch1d<1024>((const Inst1D *)ws, n, counter, s, nt_max); break;
+    }
+    if (r) return r;
+    std::vector<Stat1D> hs(n);
+    for (int i = 0; i < n; ++i)
+        UPPE_HIP(hipMemcpyAsync(&hs[i], ins[i].st, sizeof(Stat1D), hipMemcpyDeviceToHost, s));
+    UPPE_HIP(hipStreamSynchronize(s));
+    int bad = 0;
+    for (int i = 0; i < n; ++i) {
+        uppe1d_status &o = st[i];
+        o.z = hs[i].z;
+        o.h = hs[i].h;
+        o.err = hs[i].err;
+        o.attempts = hs[i].attempts;
+        o.accepted = hs[i].accepted;
+        o.rejected = hs[i].rejected;
+        o.forced = hs[i].forced;
+        o.nhist = hs[i].nhist;
+        o.done = hs[i].done;
+        o.nonfinite = hs[i].nonfinite;
+        o.fsal_hits = hs[i].fsal_hits;
+        if (hs[i].nonfinite) bad = 1;
+    }
+    if (bad) {
+        uppe::set_error("uppe1d: error norm became non-finite (NaN/Inf field)");
+        return UPPE_ENONFINITE;
+    }
+    return UPPE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int uppe1d_workspace_size(const uppe1d_desc *d, size_t *bytes) {
+    UPPE_CHECK(d && bytes, "uppe1d_workspace_size: null argument");
+    *bytes = align256(sizeof(Inst1D)) + 256 + inst_bytes(d->nt);
+    return UPPE_OK;
+}
+
+int uppe1d_propagate(const uppe1d_desc *d, const double *D_re, const double *D_im,
+                     const double *uppe_factor, const double *H_re, const double *H_im,
+                     const void *a0, void *workspace, void *hist, void *zhist, void *stream,
+                     uppe1d_status *st) {
+    UPPE_CHECK(d != nullptr, "uppe1d_propagate: null descriptor");
+    return run_batch(1, d, D_re, D_im, uppe_factor, H_re, H_im, a0, workspace, hist, zhist, 0,
+                     stream, st);
+}
+
+int uppe1d_propagate_batched(int n, const uppe1d_desc *descs, const double *D_re,
+                             const double *D_im, const double *uppe_factor,
+                             const double *H_re, const double *H_im, const void *a0,
+                             void *workspace, void *hist, void *zhist, int threads,
+                             void *stream, uppe1d_status *st) {
+    return run_batch(n, descs, D_re, D_im, uppe_factor, H_re, H_im, a0, workspace, hist, zhist,
+                     threads, stream, st);
+}
+
+}  // extern "C"
