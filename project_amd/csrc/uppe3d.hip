@@ -1312,12 +1312,12 @@ int uppe3d_attempt_timed(uppe3d_plan *p, int n, void *stream, float *kernel_ms,
         for (int a = 0; a < n; ++a)
             for (int i = 0; i < 11; ++i) {
                 float ms = 0.0f;
-                hipEventElapsedTime(&ms, ev[12 * (size_t)a + i], ev[12 * (size_t)a + i + 1]);
+                (void)hipEventElapsedTime(&ms, ev[12 * (size_t)a + i], ev[12 * (size_t)a + i + 1]);
                 kernel_ms[i] += ms;
             }
-        hipEventElapsedTime(total_ms, ev[0], ev.back());
+        (void)hipEventElapsedTime(total_ms, ev[0], ev.back());
     }
-    for (size_t i = 0; i < made; ++i) hipEventDestroy(ev[i]);
+    for (size_t i = 0; i < made; ++i) (void)hipEventDestroy(ev[i]);
     return rc;
 }
 
