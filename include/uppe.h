@@ -125,32 +125,51 @@ typedef struct {
 } uppe1d_desc;
 
 typedef struct {
-    double z, h, err;
-    int attempts, accepted, rejected, forced, nhist, done, nonfinite, fsal_hits;
+    double z, h, err;      /* final position, next step, last error norm */
+    int attempts, accepted, rejected, forced, nhist, done, nonfinite, fsal_hits, overflow;
 } uppe1d_status;
 
-/* Device bytes needed by one instance for its tables and history. */
+/* Optional recording of the accepted step sequence and checkpoints (for the adjoint):
+ * hs[k] = size of accepted step k (k < hs_cap); ckpt[j] = state A(omega) before accepted step
+ * j*ckpt_every (j < ckpt_cap), complex64 [ckpt_cap][nt]. */
+typedef struct {
+    double *hs;
+    int hs_cap;
+    void *ckpt;
+    int ckpt_every, ckpt_cap;
+} uppe1d_record;
+
+/* Device bytes needed by one instance (tables, state, status). */
 int uppe1d_workspace_size(const uppe1d_desc *d, size_t *bytes);
 
-/* Propagates one pulse (erk43ip_method.py:288-369).  D (complex, nt), uppe factor (nt),
- * H_R (complex, nt, or NULL) are FP64 host arrays in FFT order.  a0: complex64 (nt) device.
- * hist: complex64 (max_hist, nt) device; zhist: float64 (max_hist) device.  Snapshot 0 is A0;
- * the history rule of the reference appends after accepted steps.  st receives the result. */
+/* Propagates one pulse (erk43ip_method.py:288-369).  D (complex, nt), uppe factor (nt) and
+ * H_R (complex, nt, or NULL) are FP64 host arrays in FFT order.  a0: complex64 (nt), time
+ * domain, device.  hist: complex64 (max_hist, nt) device, zhist: float64 (max_hist) device:
+ * snapshot 0 is A0, then the reference history rule ("append while fewer than 1000 entries or
+ * when z advanced by more than L/500"); once the buffer is full further snapshots are dropped
+ * and st->overflow is set.  out_w (optional): complex64 (nt), final A(omega).  rec: optional
+ * step recording.  Synchronises the stream. */
 int uppe1d_propagate(const uppe1d_desc *d, const double *D_re, const double *D_im,
                      const double *uppe_factor, const double *H_re, const double *H_im,
-                     const void *a0, void *workspace, void *hist, void *zhist, void *stream,
-                     uppe1d_status *st);
+                     const void *a0, void *workspace, void *hist, void *zhist, void *out_w,
+                     const uppe1d_record *rec, void *stream, uppe1d_status *st);
 
-/* Batched: n instances of uneven nt in one persistent launch, each with its own adaptive
- * step.  descs[i], tables concatenated in order (offsets[i] = sum of nt before i), a0 and
- * hist likewise concatenated (hist: max_hist[i]*nt[i] per instance), zhist max_hist[i]
- * per instance.  threads: workgroup size chosen by the autotuner (256, 512 or 1024; 0 =
- * library default).  Status for each instance in st[i]. */
-int uppe1d_propagate_batched(int n, const uppe1d_desc *descs, const double *D_re,
-                             const double *D_im, const double *uppe_factor,
-                             const double *H_re, const double *H_im, const void *a0,
-                             void *workspace, void *hist, void *zhist, int threads,
-                             void *stream, uppe1d_status *st);
+/* Batched sweep: n instances of uneven nt, each with its own adaptive step, run by one
+ * persistent kernel (workgroups take instances from a device counter).  Arrays are
+ * concatenated in instance order: tables and a0/out_w by nt, hist by max_hist*nt, zhist by
+ * max_hist, rec by instance.  prepare uploads tables (synchronises); launch enqueues only
+ * kernels (capturable in a hipGraph) for instances [first, first+count) with `threads` per
+ * workgroup (256/512/1024; need nt/threads in 1..8) and an LDS capacity of lds_cap points
+ * (2048 or 8192), using counter `slot` (0..29, one per concurrent launch); status reads the
+ * per-instance results (synchronises). */
+int uppe1d_batch_workspace_size(int n, const uppe1d_desc *descs, size_t *bytes);
+int uppe1d_batch_prepare(int n, const uppe1d_desc *descs, const double *D_re,
+                         const double *D_im, const double *uppe_factor, const double *H_re,
+                         const double *H_im, const void *a0, void *workspace, void *hist,
+                         void *zhist, void *out_w, const uppe1d_record *rec, void *stream);
+int uppe1d_batch_launch(void *workspace, int n, int first, int count, int threads, int lds_cap,
+                        int slot, void *stream);
+int uppe1d_batch_status(void *workspace, int n, void *stream, uppe1d_status *st);
 
 const char *uppe_last_error(void);
 

@@ -41,7 +41,13 @@ class Uppe1DDesc(ctypes.Structure):
 class Uppe1DStatus(ctypes.Structure):
     _fields_ = [("z", c_double), ("h", c_double), ("err", c_double), ("attempts", c_int),
                 ("accepted", c_int), ("rejected", c_int), ("forced", c_int), ("nhist", c_int),
-                ("done", c_int), ("nonfinite", c_int), ("fsal_hits", c_int)]
+                ("done", c_int), ("nonfinite", c_int), ("fsal_hits", c_int),
+                ("overflow", c_int)]
+
+
+class Uppe1DRecord(ctypes.Structure):
+    _fields_ = [("hs", c_void_p), ("hs_cap", c_int), ("ckpt", c_void_p),
+                ("ckpt_every", c_int), ("ckpt_cap", c_int)]
 
 
 # name -> (restype, argtypes); must match include/uppe.h
@@ -64,10 +70,13 @@ SIGNATURES = {
     "uppe3d_state_ptr": (c_int, [c_void_p, c_void_p, P(c_void_p)]),
     "uppe1d_workspace_size": (c_int, [P(Uppe1DDesc), P(c_size_t)]),
     "uppe1d_propagate": (c_int, [P(Uppe1DDesc)] + [c_void_p] * 5
-                         + [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, P(Uppe1DStatus)]),
-    "uppe1d_propagate_batched": (c_int, [c_int, P(Uppe1DDesc)] + [c_void_p] * 5
-                                 + [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
-                                    P(Uppe1DStatus)]),
+                         + [c_void_p] * 5 + [P(Uppe1DRecord), c_void_p, P(Uppe1DStatus)]),
+    "uppe1d_batch_workspace_size": (c_int, [c_int, P(Uppe1DDesc), P(c_size_t)]),
+    "uppe1d_batch_prepare": (c_int, [c_int, P(Uppe1DDesc)] + [c_void_p] * 5
+                             + [c_void_p] * 5 + [P(Uppe1DRecord), c_void_p]),
+    "uppe1d_batch_launch": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int,
+                                    c_void_p]),
+    "uppe1d_batch_status": (c_int, [c_void_p, c_int, c_void_p, P(Uppe1DStatus)]),
 }
 
 _lib = None

@@ -47,6 +47,10 @@ struct Inst1D {
     float2 *A, *a5, *v;    // state (omega), FSAL term, IP-transformed state
     float2 *hist;          // [max_hist][nt], time domain
     double *zhist;         // [max_hist]
+    float2 *outw;          // final state A(omega) (nt), or null
+    double *hs;            // accepted step sizes (record for the adjoint), or null
+    float2 *ckpt;          // states A(omega) before accepted steps 0, C, 2C, ... ([ckpt_cap][nt])
+    int hs_cap, ckpt_every, ckpt_cap, pad_;
     Stat1D *st;
 };
 
@@ -429,8 +433,9 @@ __device__ void run_instance(const Inst1D &P, float2 *bufA, float2 *bufB, double
     while (z < P.L) {
         if (P.max_attempts > 0 && attempts >= P.max_attempts) break;
         if (z + h > P.L) h = P.L - z;
-        const int slot = nh < P.max_hist ? nh : (P.max_hist > 0 ? P.max_hist - 1 : -1);
+        const int slot = nh < P.max_hist ? nh : -1;   // no overwrite once the buffer is full
         float2 *snap = slot >= 0 ? P.hist + (size_t)slot * n : nullptr;
+        double h_used = h;
         err = attempt<T, NMAX>(P, h, A4, a5, bufA, bufB, scan, tid, snap);
         ++attempts;
         if (!(err == err) || isinf(err)) {
@@ -454,6 +459,7 @@ __device__ void run_instance(const Inst1D &P, float2 *bufA, float2 *bufB, double
             h = hn > P.min_step ? hn : P.min_step;
             if (h <= P.min_step) {
                 // forced step at min_step (erk43ip_method.py:344-367)
+                h_used = h;
                 err = attempt<T, NMAX>(P, h, A4, a5, bufA, bufB, scan, tid, snap);
                 ++attempts;
                 ++forced;
@@ -466,6 +472,22 @@ __device__ void run_instance(const Inst1D &P, float2 *bufA, float2 *bufB, double
                 accept = true;
             }
         }
+        if (accept && P.hs) {
+            // frozen step sequence and checkpoints for the adjoint (state before the step)
+            const int k = acc + forced - 1;
+            if (k < P.hs_cap) {
+                if (tid == 0) P.hs[k] = h_used;
+                if (P.ckpt && P.ckpt_every > 0 && k % P.ckpt_every == 0 &&
+                    k / P.ckpt_every < P.ckpt_cap) {
+                    float2 *cp = P.ckpt + (size_t)(k / P.ckpt_every) * n;
+#pragma unroll
+                    for (int i = 0; i < MAXE; ++i)
+                        if (i < E) cp[tid + T * i] = P.A[tid + T * i];
+                }
+            } else {
+                overflow = 1;
+            }
+        }
         if (accept) {
             // history rule of erk43ip_method.py:327-329
             record = (nh < 1000) || (z - zlast > P.L / 500.0);
@@ -476,14 +498,22 @@ __device__ void run_instance(const Inst1D &P, float2 *bufA, float2 *bufB, double
                     P.a5[tid + T * i] = a5[i];
                 }
             __syncthreads();
-            if (record && slot >= 0) {
-                if (tid == 0) P.zhist[slot] = z;
-                zlast = z;
-                if (nh < P.max_hist) ++nh;
-                else overflow = 1;
+            if (record) {
+                if (slot >= 0) {
+                    if (tid == 0) P.zhist[slot] = z;
+                    zlast = z;
+                    ++nh;
+                } else {
+                    overflow = 1;
+                }
             }
             __syncthreads();
         }
+    }
+    if (P.outw) {
+#pragma unroll
+        for (int i = 0; i < MAXE; ++i)
+            if (i < E) P.outw[tid + T * i] = P.A[tid + T * i];
     }
     if (tid == 0) {
         Stat1D s;
@@ -497,7 +527,7 @@ __device__ void run_instance(const Inst1D &P, float2 *bufA, float2 *bufB, double
         s.nhist = nh;
         s.done = (z >= P.L) ? 1 : 0;
         s.nonfinite = nonfinite;
-        s.fsal_hits = attempts > 0 ? attempts - 0 : 0;
+        s.fsal_hits = attempts;   // every attempt reuses N(A) (computed once at the start)
         s.hist_overflow = overflow;
         s.pad_ = 0;
         *P.st = s;
@@ -507,8 +537,10 @@ __device__ void run_instance(const Inst1D &P, float2 *bufA, float2 *bufB, double
 
 __global__ void zero_counter(int *c) { *c = 0; }
 
+// Persistent kernel: workgroups take instances [first, first + count) from a device counter
+// until none is left.  An instance longer than the LDS capacity NMAX is marked as failed.
 template <int T, int NMAX>
-__global__ __launch_bounds__(T) void uppe1d_kernel(const Inst1D *__restrict__ insts, int n_inst,
+__global__ __launch_bounds__(T) void uppe1d_kernel(const Inst1D *__restrict__ insts, int count,
                                                    int *counter) {
     __shared__ float2 bufA[NMAX], bufB[NMAX];
     __shared__ double scan[64];
@@ -520,8 +552,16 @@ __global__ __launch_bounds__(T) void uppe1d_kernel(const Inst1D *__restrict__ in
         __syncthreads();
         const int ii = next;
         __syncthreads();
-        if (ii >= n_inst) break;
+        if (ii >= count) break;
         const Inst1D P = insts[ii];
+        if (P.nt > NMAX || P.nt < T || P.nt % T != 0 || P.nt / T > MAXE) {
+            if (tid == 0) {
+                P.st->done = 0;
+                P.st->nonfinite = 0;
+                P.st->attempts = -1;   // configuration rejected on the device
+            }
+            continue;
+        }
         run_instance<T, NMAX>(P, bufA, bufB, scan, bc, tid);
     }
 }
@@ -529,14 +569,26 @@ __global__ __launch_bounds__(T) void uppe1d_kernel(const Inst1D *__restrict__ in
 }  // namespace
 
 // ---------------------------------------------------------------------------------------
-// Host side
+// Host side.  Workspace layout: [header][Inst1D x n][per-instance tables and state].
 namespace {
 
-int check1d(const uppe1d_desc &d, int threads) {
+struct Header1D {
+    int n, pad_;
+    int counters[30];   // one per concurrent launch slot
+};
+
+size_t align256(size_t v) { return (v + 255) / 256 * 256; }
+
+size_t inst_bytes(int nt) {
+    return align256(16 * (size_t)nt) + align256(4 * (size_t)nt) + 5 * align256(8 * (size_t)nt) +
+           align256(sizeof(Stat1D));
+}
+
+size_t header_bytes(int n) { return align256(sizeof(Header1D)) + align256(sizeof(Inst1D) * n); }
+
+int check1d(const uppe1d_desc &d) {
     UPPE_CHECK(d.nt >= 256 && d.nt <= 8192 && (d.nt & (d.nt - 1)) == 0,
                "uppe1d: nt = %d unsupported (power of two in [256, 8192])", d.nt);
-    UPPE_CHECK(d.nt % threads == 0 && d.nt / threads <= MAXE && d.nt >= threads,
-               "uppe1d: nt = %d needs %d..%d threads (got %d)", d.nt, d.nt / MAXE, d.nt, threads);
     UPPE_CHECK(d.tol > 0.0 && d.max_step > 0.0 && d.min_step >= 0.0 && d.L >= 0.0,
                "uppe1d: bad tolerance / step arguments");
     UPPE_CHECK(!d.use_plasma || (d.K >= 1 && d.area > 0.0 && d.rho_nt > 0.0 && d.tau_rec > 0.0),
@@ -544,47 +596,26 @@ int check1d(const uppe1d_desc &d, int threads) {
     return UPPE_OK;
 }
 
-size_t align256(size_t v) { return (v + 255) / 256 * 256; }
-
-// per-instance workspace: D_re, D_im (f64), uppe (f32), H, tw (c64), A, a5, v (c64), status
-size_t inst_bytes(int nt) {
-    return align256(16 * (size_t)nt) + align256(4 * (size_t)nt) + 5 * align256(8 * (size_t)nt) +
-           align256(sizeof(Stat1D));
-}
-
-int default_threads(int nt) { return nt <= 2048 ? 256 : nt / MAXE; }
-
-template <int T>
-int launch1d(const Inst1D *d_insts, int n, int *counter, hipStream_t s, int nt_max) {
-    const int grid = n;
-    if (nt_max <= 2048) uppe1d_kernel<T, 2048><<<grid, T, 0, s>>>(d_insts, n, counter);
-    else uppe1d_kernel<T, 8192><<<grid, T, 0, s>>>(d_insts, n, counter);
-    UPPE_HIP(hipGetLastError());
-    return UPPE_OK;
-}
-
-int run_batch(int n, const uppe1d_desc *descs, const double *D_re, const double *D_im,
-              const double *uppe_factor, const double *H_re, const double *H_im, const void *a0,
-              void *workspace, void *hist, void *zhist, int threads, void *stream,
-              uppe1d_status *st) {
-    UPPE_CHECK(n > 0 && descs && D_re && D_im && uppe_factor && a0 && workspace && st,
+int prepare(int n, const uppe1d_desc *descs, const double *D_re, const double *D_im,
+            const double *uppe_factor, const double *H_re, const double *H_im, const void *a0,
+            void *workspace, void *hist, void *zhist, void *out_w, const uppe1d_record *rec,
+            hipStream_t s) {
+    UPPE_CHECK(n > 0 && descs && D_re && D_im && uppe_factor && a0 && workspace,
                "uppe1d: null argument");
-    int nt_max = 0;
-    for (int i = 0; i < n; ++i) nt_max = descs[i].nt > nt_max ? descs[i].nt : nt_max;
-    if (threads <= 0) threads = default_threads(nt_max);
-    UPPE_CHECK(threads == 256 || threads == 512 || threads == 1024,
-               "uppe1d: threads must be 256, 512 or 1024");
     for (int i = 0; i < n; ++i) {
-        int r = check1d(descs[i], threads);
+        int r = check1d(descs[i]);
         if (r) return r;
         UPPE_CHECK(descs[i].max_hist <= 0 || (hist && zhist), "uppe1d: history buffers missing");
     }
-    hipStream_t s = (hipStream_t)stream;
     char *ws = (char *)workspace;
     std::vector<Inst1D> ins(n);
-    size_t off = align256(sizeof(Inst1D) * n) + 256;   // device copy of the descriptors + counter
-    size_t toff = 0, hoff = 0, zoff = 0;
-    const float2 *a0c = (const float2 *)a0;
+    size_t total_nt = 0;
+    for (int i = 0; i < n; ++i) total_nt += descs[i].nt;
+    // all tables of all instances are staged in one host buffer and copied once
+    size_t tables = 0;
+    for (int i = 0; i < n; ++i) tables += inst_bytes(descs[i].nt);
+    std::vector<char> stage(tables, 0);
+    size_t off = header_bytes(n), toff = 0, hoff = 0, zoff = 0, loc = 0;
     for (int i = 0; i < n; ++i) {
         const uppe1d_desc &d = descs[i];
         const int nt = d.nt;
@@ -620,83 +651,128 @@ int run_batch(int n, const uppe1d_desc *descs, const double *D_re, const double 
             P.e_rec = exp(gdt);
             P.f_rec = fabs(gdt) < 1e-5 ? P.dt * (1.0 + 0.5 * gdt) : (P.e_rec - 1.0) / (-P.r_rec);
         }
-        // tables
-        char *base = ws + off;
+        char *hb = stage.data() + loc;
+        char *db = ws + off;
         size_t o = 0;
-        std::vector<double> dd(2 * (size_t)nt);
+        double *dd = (double *)(hb + o);
         for (int k = 0; k < nt; ++k) {
             dd[k] = D_re[toff + k];
             dd[nt + k] = D_im[toff + k];
         }
-        UPPE_HIP(hipMemcpyAsync(base + o, dd.data(), 16 * (size_t)nt, hipMemcpyHostToDevice, s));
-        P.D_re = (const double *)(base + o);
-        P.D_im = (const double *)(base + o) + nt;
+        P.D_re = (const double *)(db + o);
+        P.D_im = (const double *)(db + o) + nt;
         o += align256(16 * (size_t)nt);
-        std::vector<float> uf(nt);
+        float *uf = (float *)(hb + o);
         for (int k = 0; k < nt; ++k) uf[k] = (float)uppe_factor[toff + k];
-        UPPE_HIP(hipMemcpyAsync(base + o, uf.data(), 4 * (size_t)nt, hipMemcpyHostToDevice, s));
-        P.uppe = (const float *)(base + o);
+        P.uppe = (const float *)(db + o);
         o += align256(4 * (size_t)nt);
-        std::vector<float2> hv(nt), tv(nt);
-        for (int k = 0; k < nt; ++k) {
+        float2 *hv = (float2 *)(hb + o);
+        for (int k = 0; k < nt; ++k)
             hv[k] = P.use_raman ? make_float2((float)H_re[toff + k], (float)H_im[toff + k])
                                 : make_float2(1.0f, 0.0f);
-            const double a = -2.0 * M_PI * (double)k / (double)nt;
-            tv[k] = make_float2((float)cos(a), (float)sin(a));
+        P.H = (const float2 *)(db + o);
+        o += align256(8 * (size_t)nt);
+        float2 *tv = (float2 *)(hb + o);
+        for (int k = 0; k < nt; ++k) {
+            const double ang = -2.0 * M_PI * (double)k / (double)nt;
+            tv[k] = make_float2((float)cos(ang), (float)sin(ang));
         }
-        UPPE_HIP(hipMemcpyAsync(base + o, hv.data(), 8 * (size_t)nt, hipMemcpyHostToDevice, s));
-        P.H = (const float2 *)(base + o);
+        P.tw = (const float2 *)(db + o);
         o += align256(8 * (size_t)nt);
-        UPPE_HIP(hipMemcpyAsync(base + o, tv.data(), 8 * (size_t)nt, hipMemcpyHostToDevice, s));
-        P.tw = (const float2 *)(base + o);
+        P.A = (float2 *)(db + o);
         o += align256(8 * (size_t)nt);
-        P.A = (float2 *)(base + o);
+        P.a5 = (float2 *)(db + o);
         o += align256(8 * (size_t)nt);
-        P.a5 = (float2 *)(base + o);
+        P.v = (float2 *)(db + o);
         o += align256(8 * (size_t)nt);
-        P.v = (float2 *)(base + o);
-        o += align256(8 * (size_t)nt);
-        P.st = (Stat1D *)(base + o);
+        P.st = (Stat1D *)(db + o);
         o += align256(sizeof(Stat1D));
         off += o;
-        // the host vectors above must outlive the async copies
-        UPPE_HIP(hipStreamSynchronize(s));
-        P.a0 = a0c + toff;
+        loc += o;
+        P.a0 = (const float2 *)a0 + toff;
         P.hist = P.max_hist > 0 ? (float2 *)hist + hoff : nullptr;
         P.zhist = P.max_hist > 0 ? (double *)zhist + zoff : nullptr;
+        P.outw = out_w ? (float2 *)out_w + toff : nullptr;
+        if (rec) {
+            P.hs = rec[i].hs;
+            P.hs_cap = rec[i].hs_cap;
+            P.ckpt = (float2 *)rec[i].ckpt;
+            P.ckpt_every = rec[i].ckpt_every;
+            P.ckpt_cap = rec[i].ckpt_cap;
+        }
         toff += nt;
         hoff += (size_t)P.max_hist * nt;
         zoff += P.max_hist;
     }
-    UPPE_HIP(hipMemcpyAsync(ws, ins.data(), sizeof(Inst1D) * n, hipMemcpyHostToDevice, s));
-    int *counter = (int *)(ws + align256(sizeof(Inst1D) * n));
+    Header1D h;
+    memset(&h, 0, sizeof h);
+    h.n = n;
+    UPPE_HIP(hipMemcpyAsync(ws, &h, sizeof h, hipMemcpyHostToDevice, s));
+    UPPE_HIP(hipMemcpyAsync(ws + align256(sizeof(Header1D)), ins.data(), sizeof(Inst1D) * n,
+                            hipMemcpyHostToDevice, s));
+    UPPE_HIP(hipMemcpyAsync(ws + header_bytes(n), stage.data(), tables, hipMemcpyHostToDevice, s));
+    UPPE_HIP(hipStreamSynchronize(s));   // the staged host buffers go out of scope
+    return UPPE_OK;
+}
+
+template <int T>
+int launch_t(const Inst1D *insts, int count, int *counter, int lds_cap, hipStream_t s) {
     zero_counter<<<1, 1, 0, s>>>(counter);   // a kernel, not a memset node (graph capture)
-    int r;
+    if (lds_cap <= 2048) uppe1d_kernel<T, 2048><<<count, T, 0, s>>>(insts, count, counter);
+    else uppe1d_kernel<T, 8192><<<count, T, 0, s>>>(insts, count, counter);
+    UPPE_HIP(hipGetLastError());
+    return UPPE_OK;
+}
+
+int launch(void *workspace, int n, int first, int count, int threads, int lds_cap, int slot,
+           hipStream_t s) {
+    UPPE_CHECK(workspace && first >= 0 && count > 0 && first + count <= n,
+               "uppe1d_batch_launch: bad instance range");
+    UPPE_CHECK(threads == 256 || threads == 512 || threads == 1024,
+               "uppe1d_batch_launch: threads must be 256, 512 or 1024");
+    UPPE_CHECK(slot >= 0 && slot < 30, "uppe1d_batch_launch: slot must be in [0, 30)");
+    char *ws = (char *)workspace;
+    const Inst1D *insts = (const Inst1D *)(ws + align256(sizeof(Header1D))) + first;
+    int *counter = ((Header1D *)ws)->counters + slot;
     switch (threads) {
-        case 256: r = launch1d<256>((const Inst1D *)ws, n, counter, s, nt_max); break;
-        case 512: r = launch1d<512>((const Inst1D *)ws, n, counter, s, nt_max); break;
-        default: r = launch1d<1024>((const Inst1D *)ws, n, counter, s, nt_max); break;
+        case 256: return launch_t<256>(insts, count, counter, lds_cap, s);
+        case 512: return launch_t<512>(insts, count, counter, lds_cap, s);
+        default: return launch_t<1024>(insts, count, counter, lds_cap, s);
     }
-    if (r) return r;
+}
+
+int status(void *workspace, int n, hipStream_t s, uppe1d_status *st) {
+    UPPE_CHECK(workspace && st && n > 0, "uppe1d_batch_status: null argument");
+    std::vector<Inst1D> ins(n);
+    char *ws = (char *)workspace;
+    UPPE_HIP(hipMemcpyAsync(ins.data(), ws + align256(sizeof(Header1D)), sizeof(Inst1D) * n,
+                            hipMemcpyDeviceToHost, s));
+    UPPE_HIP(hipStreamSynchronize(s));
     std::vector<Stat1D> hs(n);
     for (int i = 0; i < n; ++i)
         UPPE_HIP(hipMemcpyAsync(&hs[i], ins[i].st, sizeof(Stat1D), hipMemcpyDeviceToHost, s));
     UPPE_HIP(hipStreamSynchronize(s));
-    int bad = 0;
+    int bad = 0, rejected_cfg = 0;
     for (int i = 0; i < n; ++i) {
         uppe1d_status &o = st[i];
         o.z = hs[i].z;
         o.h = hs[i].h;
         o.err = hs[i].err;
         o.attempts = hs[i].attempts;
-        o.accepted = hs[i].accepted;
+        o.accepted = hs[i].accepted + hs[i].forced;
         o.rejected = hs[i].rejected;
         o.forced = hs[i].forced;
         o.nhist = hs[i].nhist;
         o.done = hs[i].done;
         o.nonfinite = hs[i].nonfinite;
         o.fsal_hits = hs[i].fsal_hits;
+        o.overflow = hs[i].hist_overflow;
         if (hs[i].nonfinite) bad = 1;
+        if (hs[i].attempts < 0) rejected_cfg = 1;
+    }
+    if (rejected_cfg) {
+        uppe::set_error("uppe1d: an instance does not fit the launch (nt vs threads / LDS cap)");
+        return UPPE_EINVAL;
     }
     if (bad) {
         uppe::set_error("uppe1d: error norm became non-finite (NaN/Inf field)");
@@ -705,32 +781,55 @@ int run_batch(int n, const uppe1d_desc *descs, const double *D_re, const double 
     return UPPE_OK;
 }
 
+int default_threads(int nt) { return nt <= 2048 ? 256 : nt / MAXE; }
+
 }  // namespace
 
 extern "C" {
 
 int uppe1d_workspace_size(const uppe1d_desc *d, size_t *bytes) {
     UPPE_CHECK(d && bytes, "uppe1d_workspace_size: null argument");
-    *bytes = align256(sizeof(Inst1D)) + 256 + inst_bytes(d->nt);
+    *bytes = header_bytes(1) + inst_bytes(d->nt);
     return UPPE_OK;
+}
+
+int uppe1d_batch_workspace_size(int n, const uppe1d_desc *descs, size_t *bytes) {
+    UPPE_CHECK(n > 0 && descs && bytes, "uppe1d_batch_workspace_size: null argument");
+    size_t b = header_bytes(n);
+    for (int i = 0; i < n; ++i) b += inst_bytes(descs[i].nt);
+    *bytes = b;
+    return UPPE_OK;
+}
+
+int uppe1d_batch_prepare(int n, const uppe1d_desc *descs, const double *D_re,
+                         const double *D_im, const double *uppe_factor, const double *H_re,
+                         const double *H_im, const void *a0, void *workspace, void *hist,
+                         void *zhist, void *out_w, const uppe1d_record *rec, void *stream) {
+    return prepare(n, descs, D_re, D_im, uppe_factor, H_re, H_im, a0, workspace, hist, zhist,
+                   out_w, rec, (hipStream_t)stream);
+}
+
+int uppe1d_batch_launch(void *workspace, int n, int first, int count, int threads, int lds_cap,
+                        int slot, void *stream) {
+    return launch(workspace, n, first, count, threads, lds_cap, slot, (hipStream_t)stream);
+}
+
+int uppe1d_batch_status(void *workspace, int n, void *stream, uppe1d_status *st) {
+    return status(workspace, n, (hipStream_t)stream, st);
 }
 
 int uppe1d_propagate(const uppe1d_desc *d, const double *D_re, const double *D_im,
                      const double *uppe_factor, const double *H_re, const double *H_im,
-                     const void *a0, void *workspace, void *hist, void *zhist, void *stream,
-                     uppe1d_status *st) {
-    UPPE_CHECK(d != nullptr, "uppe1d_propagate: null descriptor");
-    return run_batch(1, d, D_re, D_im, uppe_factor, H_re, H_im, a0, workspace, hist, zhist, 0,
-                     stream, st);
-}
-
-int uppe1d_propagate_batched(int n, const uppe1d_desc *descs, const double *D_re,
-                             const double *D_im, const double *uppe_factor,
-                             const double *H_re, const double *H_im, const void *a0,
-                             void *workspace, void *hist, void *zhist, int threads,
-                             void *stream, uppe1d_status *st) {
-    return run_batch(n, descs, D_re, D_im, uppe_factor, H_re, H_im, a0, workspace, hist, zhist,
-                     threads, stream, st);
+                     const void *a0, void *workspace, void *hist, void *zhist, void *out_w,
+                     const uppe1d_record *rec, void *stream, uppe1d_status *st) {
+    UPPE_CHECK(d != nullptr && st != nullptr, "uppe1d_propagate: null argument");
+    hipStream_t s = (hipStream_t)stream;
+    int r = prepare(1, d, D_re, D_im, uppe_factor, H_re, H_im, a0, workspace, hist, zhist, out_w,
+                    rec, s);
+    if (r) return r;
+    r = launch(workspace, 1, 0, 1, default_threads(d->nt), d->nt, 0, s);
+    if (r) return r;
+    return status(workspace, 1, s, st);
 }
 
 }  // extern "C"

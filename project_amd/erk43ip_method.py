@@ -224,8 +224,8 @@ class ERK43IP_UPPE:
             ctypes.byref(d), _lib.host_ptr(Dre), _lib.host_ptr(Dim), _lib.host_ptr(uf),
             _lib.host_ptr(Hre) if Hre is not None else None,
             _lib.host_ptr(Him) if Him is not None else None,
-            _lib.ptr(a0), _lib.ptr(ws), _lib.ptr(hist), _lib.ptr(zh), _lib.stream_handle(),
-            ctypes.byref(st))
+            _lib.ptr(a0), _lib.ptr(ws), _lib.ptr(hist), _lib.ptr(zh), None, None,
+            _lib.stream_handle(), ctypes.byref(st))
         if rc == -3:
             raise FloatingPointError(lib.uppe_last_error().decode())
         _lib.check(rc, "uppe1d_propagate")

@@ -45,3 +45,37 @@ def pulse1d(nt=4096, window=400e-15, energy=1e-6, fwhm=30e-15, noise=1e-6, seed=
         a = a + noise * np.abs(a).max() * (rng.standard_normal(nt)
                                            + 1j * rng.standard_normal(nt)) / np.sqrt(2)
     return t, a.astype(np.complex128)
+
+
+# -- batched 1D sweep: N_t ~ U{1024..8192}, E ~ logU[0.1, 10] uJ, GDD ~ U[-500, 500] fs^2,
+#    P ~ U[0.5, 3] bar, argon, 400 fs window, L = 1 m -------------------------------------
+def batch1d_params(n=256, seed=2, nts=(1024, 2048, 4096, 8192)):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        out.append(dict(nt=int(rng.choice(nts)),
+                        energy=float(10 ** rng.uniform(np.log10(0.1e-6), np.log10(10e-6))),
+                        gdd=float(rng.uniform(-500e-30, 500e-30)),
+                        pressure_bar=float(rng.uniform(0.5, 3.0))))
+    return out
+
+
+def argon_solver_1d(pressure_bar=1.0, beam_radius=20e-6, sigma8_per_atm=1e-130, plasma=True):
+    """The 1D argon setup of the sweep: n2 and the plasma constants scale with pressure
+    (n2 = 1e-23 P[bar], as the reference gas driver scales n2, gas.py:69-71)."""
+    from . import gas  # noqa: F401  (argon refractive-index patch)
+    from .erk43ip_method import ERK43IP_UPPE, Plasma1D
+    p_atm = pressure_bar * 1e5 / 101325.0
+    pl = Plasma1D(K=8, sigma_K=sigma8_per_atm * p_atm, rho_nt=2.45e25 * p_atm,
+                  tau_c=190e-15 / p_atm) if plasma else None
+    return ERK43IP_UPPE(material="argon_gas", n2=1e-23 * pressure_bar, beam_radius=beam_radius,
+                        center_wavelength=800e-9, use_raman=False, use_self_steepening=True,
+                        gas_P=pressure_bar * 1e5, gas_T=293.15, plasma=pl)
+
+
+def batch1d_inputs(params, seed=2):
+    inputs = []
+    for k, p in enumerate(params):
+        t, a = pulse1d(p["nt"], energy=p["energy"], GDD=p["gdd"], seed=seed * 1000 + k)
+        inputs.append((t, a))
+    return inputs
