@@ -126,17 +126,21 @@ typedef struct {
 
 typedef struct {
     double z, h, err;      /* final position, next step, last error norm */
-    int attempts, accepted, rejected, forced, nhist, done, nonfinite, fsal_hits, overflow;
+    int attempts, accepted, rejected, forced, nhist, done, nonfinite, fsal_hits;
+    int overflow;          /* bit 0: history buffer full, bit 1: step record capacity exceeded */
 } uppe1d_status;
 
 /* Optional recording of the accepted step sequence and checkpoints (for the adjoint):
  * hs[k] = size of accepted step k (k < hs_cap); ckpt[j] = state A(omega) before accepted step
- * j*ckpt_every (j < ckpt_cap), complex64 [ckpt_cap][nt]. */
+ * j*ckpt_every (j < ckpt_cap), complex64 [ckpt_cap][nt].  replay_hs (device, replay_M steps):
+ * when set, the propagation takes exactly these steps, all accepted (frozen step sequence). */
 typedef struct {
     double *hs;
     int hs_cap;
     void *ckpt;
     int ckpt_every, ckpt_cap;
+    const double *replay_hs;
+    int replay_M;
 } uppe1d_record;
 
 /* Device bytes needed by one instance (tables, state, status). */
@@ -170,6 +174,21 @@ int uppe1d_batch_prepare(int n, const uppe1d_desc *descs, const double *D_re,
 int uppe1d_batch_launch(void *workspace, int n, int first, int count, int threads, int lds_cap,
                         int slot, void *stream);
 int uppe1d_batch_status(void *workspace, int n, void *stream, uppe1d_status *st);
+
+/* Adjoint of a recorded forward propagation (reverse mode through the frozen accepted-step
+ * sequence; no counterpart in the reference).  fwd_workspace: the workspace of the forward run
+ * (uppe1d_propagate or uppe1d_batch_prepare), instance `index` of n.  hs (device, M accepted
+ * step sizes) and ckpt (device, states before steps 0, C, 2C, ...) come from its
+ * uppe1d_record.  lam: complex64 (nt) device, in: dL/dA(L) in the omega domain, out: dL/dA(0).
+ * Convention for complex gradients: dL/dRe z + i dL/dIm z.  Outputs (device, float64):
+ * gD (2 nt, re/im interleaved) = dL/dD(omega), gU (nt) = dL/d(UPPE factor), gG (1) =
+ * dL/dgamma.  The segment between checkpoints is recomputed once; the plasma term is
+ * differentiated in its affine regime (zero avalanche rate, no density clamp) and the call
+ * fails otherwise.  Synchronises the stream. */
+int uppe1d_adjoint_workspace_size(int nt, int ckpt_every, size_t *bytes);
+int uppe1d_adjoint(void *fwd_workspace, int n, int index, int M, const double *hs,
+                   const void *ckpt, int ckpt_every, void *lam, void *adj_workspace, double *gD,
+                   double *gU, double *gG, void *stream);
 
 const char *uppe_last_error(void);
 

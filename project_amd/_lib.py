@@ -47,7 +47,8 @@ class Uppe1DStatus(ctypes.Structure):
 
 class Uppe1DRecord(ctypes.Structure):
     _fields_ = [("hs", c_void_p), ("hs_cap", c_int), ("ckpt", c_void_p),
-                ("ckpt_every", c_int), ("ckpt_cap", c_int)]
+                ("ckpt_every", c_int), ("ckpt_cap", c_int), ("replay_hs", c_void_p),
+                ("replay_M", c_int)]
 
 
 # name -> (restype, argtypes); must match include/uppe.h
@@ -77,6 +78,9 @@ SIGNATURES = {
     "uppe1d_batch_launch": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int,
                                     c_void_p]),
     "uppe1d_batch_status": (c_int, [c_void_p, c_int, c_void_p, P(Uppe1DStatus)]),
+    "uppe1d_adjoint_workspace_size": (c_int, [c_int, c_int, P(c_size_t)]),
+    "uppe1d_adjoint": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int,
+                               c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
 }
 
 _lib = None

@@ -32,7 +32,7 @@ constexpr int MAXE = 8;   // elements per thread
 struct Stat1D {
     double z, h, err;
     int attempts, accepted, rejected, forced, nhist, done, nonfinite, fsal_hits;
-    int hist_overflow, pad_;
+    int hist_overflow, rec_overflow;
 };
 
 struct Inst1D {
@@ -48,6 +48,8 @@ struct Inst1D {
     float2 *hist;          // [max_hist][nt], time domain
     double *zhist;         // [max_hist]
     float2 *outw;          // final state A(omega) (nt), or null
+    const double *replay_hs;   // prescribed step sequence (every attempt accepted), or null
+    int replay_M, pad1_;
     double *hs;            // accepted step sizes (record for the adjoint), or null
     float2 *ckpt;          // states A(omega) before accepted steps 0, C, 2C, ... ([ckpt_cap][nt])
     int hs_cap, ckpt_every, ckpt_cap, pad_;
@@ -425,14 +427,15 @@ __device__ void run_instance(const Inst1D &P, float2 *bufA, float2 *bufB, double
     // controller state (thread 0 owns it; broadcast through bc[])
     double z = 0.0, h = P.max_step / 100.0, eprev = -1.0;   // eprev < 0: None
     int nh = P.max_hist > 0 ? 1 : 0, attempts = 0, acc = 0, rej = 0, forced = 0, nonfinite = 0;
-    int overflow = 0;
+    int overflow = 0, rec_overflow = 0;
     double zlast = 0.0;
     if (nh > 0 && tid == 0) P.zhist[0] = 0.0;
     double err = 0.0;
     float2 A4[MAXE], a5[MAXE];
-    while (z < P.L) {
+    while (P.replay_hs ? (acc < P.replay_M) : (z < P.L)) {
         if (P.max_attempts > 0 && attempts >= P.max_attempts) break;
-        if (z + h > P.L) h = P.L - z;
+        if (P.replay_hs) h = P.replay_hs[acc];
+        else if (z + h > P.L) h = P.L - z;
         const int slot = nh < P.max_hist ? nh : -1;   // no overwrite once the buffer is full
         float2 *snap = slot >= 0 ? P.hist + (size_t)slot * n : nullptr;
         double h_used = h;
@@ -442,7 +445,7 @@ __device__ void run_instance(const Inst1D &P, float2 *bufA, float2 *bufB, double
             nonfinite = 1;
             break;
         }
-        bool accept = err < P.tol;
+        bool accept = err < P.tol || P.replay_hs != nullptr;
         bool record = false;
         if (accept) {
             z += h;
@@ -485,7 +488,7 @@ __device__ void run_instance(const Inst1D &P, float2 *bufA, float2 *bufB, double
                         if (i < E) cp[tid + T * i] = P.A[tid + T * i];
                 }
             } else {
-                overflow = 1;
+                rec_overflow = 1;
             }
         }
         if (accept) {
@@ -498,7 +501,7 @@ __device__ void run_instance(const Inst1D &P, float2 *bufA, float2 *bufB, double
                     P.a5[tid + T * i] = a5[i];
                 }
             __syncthreads();
-            if (record) {
+            if (record && P.max_hist > 0) {
                 if (slot >= 0) {
                     if (tid == 0) P.zhist[slot] = z;
                     zlast = z;
@@ -525,11 +528,11 @@ __device__ void run_instance(const Inst1D &P, float2 *bufA, float2 *bufB, double
         s.rejected = rej;
         s.forced = forced;
         s.nhist = nh;
-        s.done = (z >= P.L) ? 1 : 0;
+        s.done = (P.replay_hs ? acc >= P.replay_M : z >= P.L) ? 1 : 0;
         s.nonfinite = nonfinite;
         s.fsal_hits = attempts;   // every attempt reuses N(A) (computed once at the start)
         s.hist_overflow = overflow;
-        s.pad_ = 0;
+        s.rec_overflow = rec_overflow;
         *P.st = s;
     }
     __syncthreads();
@@ -563,6 +566,451 @@ __global__ __launch_bounds__(T) void uppe1d_kernel(const Inst1D *__restrict__ in
             continue;
         }
         run_instance<T, NMAX>(P, bufA, bufB, scan, bc, tid);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Adjoint (reverse mode) of the frozen accepted-step sequence.  Gradient convention for a
+// complex quantity z: zb = dL/dRe z + i dL/dIm z (so y = c z gives zb = conj(c) yb and
+// y = M z gives zb = M^H yb; FFT^H = n IFFT, IFFT^H = FFT / n).
+struct Adj1D {
+    Inst1D P;                 // forward tables and scalars (P.A is used as state scratch)
+    const double *hs;         // accepted step sizes [M]
+    const float2 *ckpt;       // state A(omega) before steps 0, C, 2C, ... [ceil(M/C)][nt]
+    float2 *seg;              // recomputed segment: per step A, N(A), a2, a3, a4 [C][5][nt]
+    float2 *lam;              // in: dL/dA(L); out: dL/dA(0)      (omega, [nt])
+    float2 *tmp;              // vb, a1b, a2b, a3b                  [4][nt]
+    double *pscr;             // plasma adjoint scratch             [4][nt]
+    double *gD;               // dL/dD (re, im interleaved)         [2 nt]
+    double *gU;               // dL/dU (UPPE factor)                [nt]
+    double *gG;               // dL/dgamma                          [1]
+    int *flags;               // bit 0: plasma regime outside the affine (differentiable) scan
+    int M, C, pad0, pad1;
+};
+
+// Exclusive block scan of affine maps x -> A x + B held one per thread, in thread order
+// (forward: thread 0 first) or reverse thread order.  Returns the composition of all maps
+// before this thread applied to 0, i.e. the value entering this thread's segment.
+template <int T, bool REVERSE>
+__device__ double block_affine_entry(double A, double B, double *scan, int tid) {
+    const int lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        double Ao, Bo;
+        bool take;
+        if (!REVERSE) {
+            Ao = __shfl_up(A, off, 64);
+            Bo = __shfl_up(B, off, 64);
+            take = lane >= off;
+        } else {
+            Ao = __shfl_down(A, off, 64);
+            Bo = __shfl_down(B, off, 64);
+            take = lane + off < 64;
+        }
+        if (take) {
+            B = A * Bo + B;
+            A = A * Ao;
+        }
+    }
+    double Be;
+    if (!REVERSE) {
+        Be = __shfl_up(B, 1, 64);
+        if (lane == 0) Be = 0.0;
+    } else {
+        Be = __shfl_down(B, 1, 64);
+        if (lane == 63) Be = 0.0;
+    }
+    double Ae = !REVERSE ? __shfl_up(A, 1, 64) : __shfl_down(A, 1, 64);
+    if ((!REVERSE && lane == 0) || (REVERSE && lane == 63)) Ae = 1.0;
+    __syncthreads();
+    if ((!REVERSE && lane == 63) || (REVERSE && lane == 0)) {
+        scan[2 * wv] = A;
+        scan[2 * wv + 1] = B;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double pa = 1.0, pb = 0.0;
+        for (int k = 0; k < T / 64; ++k) {
+            const int w = REVERSE ? T / 64 - 1 - k : k;
+            const double a = scan[2 * w], b = scan[2 * w + 1];
+            scan[2 * w] = pa;
+            scan[2 * w + 1] = pb;
+            pb = a * pb + b;
+            pa = a * pa;
+        }
+    }
+    __syncthreads();
+    const double entry = Ae * scan[2 * wv + 1] + Be;
+    __syncthreads();
+    return entry;
+}
+
+// Adds the plasma-term contribution to ab (time, natural mapping) given a (time) and
+// Pb = dL/dP (time).  Affine regime only (zero avalanche rate, no clamp); otherwise flags.
+template <int T, int NMAX>
+__device__ void plasma_adjoint_1d(const Adj1D &J, const float2 (&a)[MAXE], const float2 (&Pb)[MAXE],
+                                  float2 (&ab)[MAXE], float2 *bufA, float2 *bufB, double *scan,
+                                  int tid) {
+    const Inst1D &P = J.P;
+    const int n = P.nt, E = n / T;
+    bool any = false, risk = false;
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i)
+        if (i < E) {
+            const double I = (double)cabs2(a[i]) * P.inv_area;
+            double pw = 1.0;
+            for (int k = 0; k < P.K; ++k) pw *= I;
+            const float w = (float)(P.sigma_K * pw);
+            bufA[tid + T * i] = make_float2(w, (float)I);
+            any |= (w > 0.0f);
+            risk |= ((double)w * P.f_rec > P.e_rec);
+        }
+    if (__syncthreads_or(any ? 1 : 0) == 0) return;
+    if (P.avalanche || __syncthreads_or(risk ? 1 : 0)) {
+        if (tid == 0) atomicOr(J.flags, 1);
+        return;
+    }
+    // forward densities rho_t (pre-update), thread-contiguous segments -> bufB as doubles
+    double *rho_s = (double *)bufB;
+    const int t0 = tid * E;
+    {
+        double A = 1.0, B = 0.0;
+        for (int k = 0; k < E; ++k) {
+            const double w = (double)bufA[t0 + k].x;
+            const double ak = P.e_rec - w * P.f_rec;
+            B = ak * B + w * P.rho_nt * P.f_rec;
+            A = ak * A;
+        }
+        double rho = block_affine_entry<T, false>(A, B, scan, tid);
+        for (int k = 0; k < E; ++k) {
+            rho_s[t0 + k] = rho;
+            const double w = (double)bufA[t0 + k].x;
+            double depl = (P.rho_nt - rho) / P.rho_nt;
+            if (depl < 0.0) depl = 0.0;
+            rho = rho * P.e_rec + (w * P.rho_nt * depl) * P.f_rec;
+            rho = rho > P.rho_nt ? P.rho_nt : (rho < 0.0 ? 0.0 : rho);
+        }
+    }
+    __syncthreads();
+    double *dd = J.pscr, *wb = J.pscr + n, *Ib = J.pscr + 2 * n;
+    // direct terms at every sample (natural mapping)
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i)
+        if (i < E) {
+            const int t = tid + T * i;
+            const double rho = rho_s[t];
+            const double w = (double)bufA[t].x, I = (double)bufA[t].y;
+            const double den = 2.0 * I + 1e-30;
+            const double cl = w * (P.rho_nt - rho) * P.Ui / den;
+            const double c1 = cl + P.half_sigma * rho, c2 = P.half_sigma * rho * P.w0tc;
+            ab[i] = cadd(ab[i], make_float2((float)(-c1 * Pb[i].x - c2 * Pb[i].y),
+                                            (float)(-c1 * Pb[i].y + c2 * Pb[i].x)));
+            const double c1b = -((double)Pb[i].x * a[i].x + (double)Pb[i].y * a[i].y);
+            const double c2b = (double)Pb[i].x * a[i].y - (double)Pb[i].y * a[i].x;
+            dd[t] = c1b * (P.half_sigma - w * P.Ui / den) + c2b * P.half_sigma * P.w0tc;
+            wb[t] = c1b * (P.rho_nt - rho) * P.Ui / den;
+            Ib[t] = -c1b * cl * 2.0 / den;
+        }
+    __syncthreads();
+    // reverse scan: rb_t = d_t + (e - w_t f) rb_{t+1}, rb_n = 0; wb_t += rb_{t+1} (rho_nt - rho_t) f
+    {
+        double A = 1.0, B = 0.0;   // segment map: rb_{t0} = A rb_{t0+E} + B
+        for (int k = E - 1; k >= 0; --k) {
+            const double al = P.e_rec - (double)bufA[t0 + k].x * P.f_rec;
+            B = dd[t0 + k] + al * B;
+            A = al * A;
+        }
+        double rb_next = block_affine_entry<T, true>(A, B, scan, tid);   // rb_{t0+E}
+        for (int k = E - 1; k >= 0; --k) {
+            const int t = t0 + k;
+            const double w = (double)bufA[t].x, I = (double)bufA[t].y;
+            double wbt = wb[t] + rb_next * (P.rho_nt - rho_s[t]) * P.f_rec;
+            const double rb = dd[t] + (P.e_rec - w * P.f_rec) * rb_next;
+            // w = sigma_K I^K  ->  dw/dI = K w / I
+            Ib[t] += (I > 0.0) ? wbt * (double)P.K * w / I : 0.0;
+            rb_next = rb;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i)
+        if (i < E) {
+            const double s = 2.0 * Ib[tid + T * i] * P.inv_area;
+            ab[i] = cadd(ab[i], make_float2((float)(s * a[i].x), (float)(s * a[i].y)));
+        }
+    __syncthreads();
+}
+
+// Fb = N^*(F; yb): adjoint of y = N(F), accumulating dL/dU (per omega) and dL/dgamma.
+template <int T, int NMAX>
+__device__ void adj_neval(const Adj1D &J, const float2 (&F)[MAXE], const float2 (&yb)[MAXE],
+                          float2 (&Fb)[MAXE], double &gG, float2 *bufA, float2 *bufB,
+                          double *scan, int tid) {
+    const Inst1D &P = J.P;
+    const int n = P.nt, E = n / T;
+    const float invn = 1.0f / (float)n;
+    const float g = (float)P.gamma;
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i)
+        if (i < E) bufA[tid + T * i] = F[i];
+    __syncthreads();
+    float2 *r = lds_fft<T, -1>(bufA, bufB, n, P.tw, tid);
+    float2 a[MAXE], pb[MAXE], Pb[MAXE], ab[MAXE];
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i)
+        if (i < E) a[i] = cscale(r[tid + T * i], invn);
+    __syncthreads();
+    double gsum = 0.0;
+    if (P.use_ss) {
+#pragma unroll
+        for (int i = 0; i < MAXE; ++i)
+            if (i < E) bufA[tid + T * i] = cscale(a[i], cabs2(a[i]));
+        __syncthreads();
+        float2 *q = lds_fft<T, 1>(bufA, bufB, n, P.tw, tid);
+        float2 Qb[MAXE];
+#pragma unroll
+        for (int i = 0; i < MAXE; ++i)
+            if (i < E) {
+                const int e = tid + T * i;
+                const float2 Q = q[e];
+                const double re = (double)yb[i].y * Q.x - (double)yb[i].x * Q.y;   // Re(conj(yb) i Q)
+                J.gU[e] += (double)P.gamma * re;
+                gsum += (double)P.uppe[e] * re;
+                const float s = g * P.uppe[e];
+                Qb[i] = make_float2(s * yb[i].y, -s * yb[i].x);   // conj(i gamma U) yb
+            }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < MAXE; ++i)
+            if (i < E) bufA[tid + T * i] = Qb[i];
+        __syncthreads();
+        float2 *r2 = lds_fft<T, -1>(bufA, bufB, n, P.tw, tid);   // n IFFT
+#pragma unroll
+        for (int i = 0; i < MAXE; ++i)
+            if (i < E) pb[i] = r2[tid + T * i];
+        __syncthreads();
+        if (P.use_plasma) {
+#pragma unroll
+            for (int i = 0; i < MAXE; ++i)
+                if (i < E) bufA[tid + T * i] = yb[i];
+            __syncthreads();
+            float2 *r3 = lds_fft<T, -1>(bufA, bufB, n, P.tw, tid);
+#pragma unroll
+            for (int i = 0; i < MAXE; ++i)
+                if (i < E) Pb[i] = r3[tid + T * i];
+            __syncthreads();
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < MAXE; ++i)
+            if (i < E) bufA[tid + T * i] = yb[i];
+        __syncthreads();
+        float2 *r2 = lds_fft<T, -1>(bufA, bufB, n, P.tw, tid);   // wb = n IFFT(yb)
+#pragma unroll
+        for (int i = 0; i < MAXE; ++i)
+            if (i < E) {
+                const float2 w = r2[tid + T * i];
+                const float2 p = cscale(a[i], cabs2(a[i]));
+                gsum += (double)w.y * p.x - (double)w.x * p.y;     // Re(conj(wb) i p)
+                pb[i] = make_float2(g * w.y, -g * w.x);           // conj(i gamma) wb
+                Pb[i] = w;
+            }
+        __syncthreads();
+    }
+    // p = |a|^2 a:  ab = 2 |a|^2 pb + conj(pb) a^2
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i)
+        if (i < E) {
+            const float m = 2.0f * cabs2(a[i]);
+            const float2 a2 = cmul(a[i], a[i]);
+            const float2 cp = make_float2(pb[i].x, -pb[i].y);
+            ab[i] = cadd(cscale(pb[i], m), cmul(cp, a2));
+        }
+    if (P.use_plasma) plasma_adjoint_1d<T, NMAX>(J, a, Pb, ab, bufA, bufB, scan, tid);
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i)
+        if (i < E) bufA[tid + T * i] = ab[i];
+    __syncthreads();
+    float2 *q2 = lds_fft<T, 1>(bufA, bufB, n, P.tw, tid);
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i)
+        if (i < E) Fb[i] = cscale(q2[tid + T * i], invn);
+    __syncthreads();
+    gG += block_sum<T>(gsum, scan + 32, tid);
+}
+
+// D-gradient of y = E z, E = exp(h/2 D):  dL/dD += (h/2) yb conj(y)
+__device__ __forceinline__ void acc_gD(double *gD, int e, double s, float2 yb, float2 y) {
+    gD[2 * e] += s * ((double)yb.x * y.x + (double)yb.y * y.y);
+    gD[2 * e + 1] += s * ((double)yb.y * y.x - (double)yb.x * y.y);
+}
+
+template <int T, int NMAX>
+__device__ void run_adjoint(const Adj1D &J, float2 *bufA, float2 *bufB, double *scan, int tid) {
+    const Inst1D &P = J.P;
+    const int n = P.nt, E = n / T;
+    const size_t nn = (size_t)n;
+    float2 lam[MAXE];
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i)
+        if (i < E) {
+            const int e = tid + T * i;
+            lam[i] = J.lam[e];
+            J.gD[2 * e] = 0.0;
+            J.gD[2 * e + 1] = 0.0;
+            J.gU[e] = 0.0;
+        }
+    double gG = 0.0;
+    float2 *vb = J.tmp, *a1b = J.tmp + nn, *a2b = J.tmp + 2 * nn, *a3b = J.tmp + 3 * nn;
+    const int nseg = (J.M + J.C - 1) / J.C;
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+        const int j0 = sg * J.C;
+        const int j1 = (j0 + J.C < J.M) ? j0 + J.C : J.M;
+        // recompute the segment forward from its checkpoint, storing A, N(A), a2, a3, a4
+        float2 A[MAXE], k1[MAXE];
+#pragma unroll
+        for (int i = 0; i < MAXE; ++i)
+            if (i < E) A[i] = J.ckpt[(size_t)sg * nn + tid + T * i];
+        neval<T, NMAX>(P, A, k1, bufA, bufB, scan, tid, nullptr);
+        for (int j = j0; j < j1; ++j) {
+            float2 *s5 = J.seg + (size_t)(j - j0) * 5 * nn;
+            const double h = J.hs[j];
+            const float hf = (float)h;
+            float2 Ef[MAXE], v[MAXE], S[MAXE], x[MAXE], y[MAXE];
+#pragma unroll
+            for (int i = 0; i < MAXE; ++i)
+                if (i < E) {
+                    const int e = tid + T * i;
+                    s5[e] = A[i];
+                    s5[nn + e] = k1[i];
+                    Ef[i] = ipexp(P, e, 0.5 * h);
+                    v[i] = cmul(Ef[i], A[i]);
+                    const float2 a1 = cmul(Ef[i], k1[i]);
+                    S[i] = caxpy(v[i], hf / 6.0f, a1);
+                    x[i] = caxpy(v[i], 0.5f * hf, a1);
+                }
+            neval<T, NMAX>(P, x, y, bufA, bufB, scan, tid, nullptr);   // a2
+#pragma unroll
+            for (int i = 0; i < MAXE; ++i)
+                if (i < E) {
+                    s5[2 * nn + tid + T * i] = y[i];
+                    S[i] = caxpy(S[i], hf / 3.0f, y[i]);
+                    x[i] = caxpy(v[i], 0.5f * hf, y[i]);
+                }
+            neval<T, NMAX>(P, x, y, bufA, bufB, scan, tid, nullptr);   // a3
+#pragma unroll
+            for (int i = 0; i < MAXE; ++i)
+                if (i < E) {
+                    s5[3 * nn + tid + T * i] = y[i];
+                    S[i] = caxpy(S[i], hf / 3.0f, y[i]);
+                    x[i] = cmul(Ef[i], caxpy(v[i], hf, y[i]));
+                }
+            neval<T, NMAX>(P, x, y, bufA, bufB, scan, tid, nullptr);   // a4
+#pragma unroll
+            for (int i = 0; i < MAXE; ++i)
+                if (i < E) {
+                    s5[4 * nn + tid + T * i] = y[i];
+                    A[i] = caxpy(cmul(Ef[i], S[i]), hf / 6.0f, y[i]);
+                }
+            if (j + 1 < j1) neval<T, NMAX>(P, A, k1, bufA, bufB, scan, tid, nullptr);
+            __syncthreads();
+        }
+        // backward through the segment
+        for (int j = j1 - 1; j >= j0; --j) {
+            const float2 *s5 = J.seg + (size_t)(j - j0) * 5 * nn;
+            const double h = J.hs[j];
+            const float hf = (float)h;
+            const double sh = 0.5 * h;
+            float2 Ef[MAXE], t1[MAXE], t2[MAXE];
+            // stage A4 = E S + h/6 a4
+#pragma unroll
+            for (int i = 0; i < MAXE; ++i)
+                if (i < E) {
+                    const int e = tid + T * i;
+                    Ef[i] = ipexp(P, e, sh);
+                    const float2 Ai = s5[e], k1i = s5[nn + e], a2 = s5[2 * nn + e],
+                                 a3 = s5[3 * nn + e];
+                    const float2 v = cmul(Ef[i], Ai), a1 = cmul(Ef[i], k1i);
+                    float2 S = caxpy(v, hf / 6.0f, a1);
+                    S = caxpy(S, hf / 3.0f, a2);
+                    S = caxpy(S, hf / 3.0f, a3);
+                    acc_gD(J.gD, e, sh, lam[i], cmul(Ef[i], S));
+                    const float2 Sb = cmulc(lam[i], Ef[i]);   // conj(E) lam
+                    vb[e] = Sb;
+                    a1b[e] = cscale(Sb, hf / 6.0f);
+                    a2b[e] = cscale(Sb, hf / 3.0f);
+                    a3b[e] = cscale(Sb, hf / 3.0f);
+                    t1[i] = cmul(Ef[i], caxpy(v, hf, a3));   // x4
+                    t2[i] = cscale(lam[i], hf / 6.0f);        // a4b
+                }
+            float2 xb[MAXE];
+            adj_neval<T, NMAX>(J, t1, t2, xb, gG, bufA, bufB, scan, tid);   // x4b
+#pragma unroll
+            for (int i = 0; i < MAXE; ++i)
+                if (i < E) {
+                    const int e = tid + T * i;
+                    acc_gD(J.gD, e, sh, xb[i], t1[i]);
+                    const float2 tb = cmulc(xb[i], Ef[i]);
+                    vb[e] = cadd(vb[e], tb);
+                    a3b[e] = caxpy(a3b[e], hf, tb);
+                    const float2 v = cmul(Ef[i], s5[e]);
+                    t1[i] = caxpy(v, 0.5f * hf, s5[2 * nn + e]);   // x3
+                    t2[i] = a3b[e];
+                }
+            adj_neval<T, NMAX>(J, t1, t2, xb, gG, bufA, bufB, scan, tid);   // x3b
+#pragma unroll
+            for (int i = 0; i < MAXE; ++i)
+                if (i < E) {
+                    const int e = tid + T * i;
+                    vb[e] = cadd(vb[e], xb[i]);
+                    a2b[e] = caxpy(a2b[e], 0.5f * hf, xb[i]);
+                    const float2 v = cmul(Ef[i], s5[e]);
+                    t1[i] = caxpy(v, 0.5f * hf, cmul(Ef[i], s5[nn + e]));   // x2
+                    t2[i] = a2b[e];
+                }
+            adj_neval<T, NMAX>(J, t1, t2, xb, gG, bufA, bufB, scan, tid);   // x2b
+#pragma unroll
+            for (int i = 0; i < MAXE; ++i)
+                if (i < E) {
+                    const int e = tid + T * i;
+                    vb[e] = cadd(vb[e], xb[i]);
+                    const float2 a1bt = caxpy(a1b[e], 0.5f * hf, xb[i]);
+                    const float2 a1 = cmul(Ef[i], s5[nn + e]);
+                    acc_gD(J.gD, e, sh, a1bt, a1);
+                    t1[i] = s5[e];                      // A
+                    t2[i] = cmulc(a1bt, Ef[i]);         // k1b
+                }
+            adj_neval<T, NMAX>(J, t1, t2, xb, gG, bufA, bufB, scan, tid);   // from k1 = N(A)
+#pragma unroll
+            for (int i = 0; i < MAXE; ++i)
+                if (i < E) {
+                    const int e = tid + T * i;
+                    const float2 v = cmul(Ef[i], s5[e]);
+                    acc_gD(J.gD, e, sh, vb[e], v);
+                    lam[i] = cadd(xb[i], cmulc(vb[e], Ef[i]));
+                }
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i)
+        if (i < E) J.lam[tid + T * i] = lam[i];
+    if (tid == 0) *J.gG = gG;
+}
+
+template <int T, int NMAX>
+__global__ __launch_bounds__(T) void uppe1d_adjoint_kernel(const Adj1D *__restrict__ jobs,
+                                                           int count) {
+    __shared__ float2 bufA[NMAX], bufB[NMAX];
+    __shared__ double scan[64];
+    for (int k = blockIdx.x; k < count; k += gridDim.x) {
+        const Adj1D J = jobs[k];
+        if (J.P.nt > NMAX || J.P.nt < T || J.P.nt % T != 0 || J.P.nt / T > MAXE) {
+            if (threadIdx.x == 0) atomicOr(J.flags, 2);
+            continue;
+        }
+        run_adjoint<T, NMAX>(J, bufA, bufB, scan, threadIdx.x);
+        __syncthreads();
     }
 }
 
@@ -693,6 +1141,10 @@ int prepare(int n, const uppe1d_desc *descs, const double *D_re, const double *D
         P.hist = P.max_hist > 0 ? (float2 *)hist + hoff : nullptr;
         P.zhist = P.max_hist > 0 ? (double *)zhist + zoff : nullptr;
         P.outw = out_w ? (float2 *)out_w + toff : nullptr;
+        if (rec && rec[i].replay_hs) {
+            P.replay_hs = rec[i].replay_hs;
+            P.replay_M = rec[i].replay_M;
+        }
         if (rec) {
             P.hs = rec[i].hs;
             P.hs_cap = rec[i].hs_cap;
@@ -766,7 +1218,7 @@ int status(void *workspace, int n, hipStream_t s, uppe1d_status *st) {
         o.done = hs[i].done;
         o.nonfinite = hs[i].nonfinite;
         o.fsal_hits = hs[i].fsal_hits;
-        o.overflow = hs[i].hist_overflow;
+        o.overflow = hs[i].hist_overflow | (hs[i].rec_overflow << 1);
         if (hs[i].nonfinite) bad = 1;
         if (hs[i].attempts < 0) rejected_cfg = 1;
     }
@@ -782,6 +1234,69 @@ int status(void *workspace, int n, hipStream_t s, uppe1d_status *st) {
 }
 
 int default_threads(int nt) { return nt <= 2048 ? 256 : nt / MAXE; }
+
+size_t adj_bytes(int nt, int ckpt_every) {
+    const size_t n = (size_t)nt;
+    return align256(sizeof(Adj1D)) + align256(5 * (size_t)ckpt_every * n * 8) +
+           align256(4 * n * 8) + align256(4 * n * 8) + align256(sizeof(int));
+}
+
+int adjoint(void *fwd_ws, int n, int index, int M, const double *hs, const void *ckpt,
+            int ckpt_every, void *lam, void *adj_ws, double *gD, double *gU, double *gG,
+            hipStream_t s) {
+    UPPE_CHECK(fwd_ws && adj_ws && hs && ckpt && lam && gD && gU && gG,
+               "uppe1d_adjoint: null argument");
+    UPPE_CHECK(index >= 0 && index < n && M >= 0 && ckpt_every > 0, "uppe1d_adjoint: bad range");
+    Inst1D P;
+    UPPE_HIP(hipMemcpyAsync(&P, (char *)fwd_ws + align256(sizeof(Header1D)) + index * sizeof(Inst1D),
+                            sizeof(Inst1D), hipMemcpyDeviceToHost, s));
+    UPPE_HIP(hipStreamSynchronize(s));
+    const size_t nt = (size_t)P.nt;
+    Adj1D J;
+    memset(&J, 0, sizeof J);
+    J.P = P;
+    J.hs = hs;
+    J.ckpt = (const float2 *)ckpt;
+    J.M = M;
+    J.C = ckpt_every;
+    char *w = (char *)adj_ws;
+    size_t o = align256(sizeof(Adj1D));
+    J.seg = (float2 *)(w + o);
+    o += align256(5 * (size_t)ckpt_every * nt * 8);
+    J.tmp = (float2 *)(w + o);
+    o += align256(4 * nt * 8);
+    J.pscr = (double *)(w + o);
+    o += align256(4 * nt * 8);
+    J.flags = (int *)(w + o);
+    J.lam = (float2 *)lam;
+    J.gD = gD;
+    J.gU = gU;
+    J.gG = gG;
+    int zero = 0;
+    UPPE_HIP(hipMemcpyAsync(J.flags, &zero, sizeof(int), hipMemcpyHostToDevice, s));
+    UPPE_HIP(hipMemcpyAsync(w, &J, sizeof J, hipMemcpyHostToDevice, s));
+    const int T = default_threads(P.nt);
+    const Adj1D *jd = (const Adj1D *)w;
+    if (M > 0) {
+        switch (T) {
+            case 256:
+                if (P.nt <= 2048) uppe1d_adjoint_kernel<256, 2048><<<1, 256, 0, s>>>(jd, 1);
+                else uppe1d_adjoint_kernel<256, 8192><<<1, 256, 0, s>>>(jd, 1);
+                break;
+            case 512: uppe1d_adjoint_kernel<512, 8192><<<1, 512, 0, s>>>(jd, 1); break;
+            default: uppe1d_adjoint_kernel<1024, 8192><<<1, 1024, 0, s>>>(jd, 1); break;
+        }
+        UPPE_HIP(hipGetLastError());
+    }
+    int flags = 0;
+    UPPE_HIP(hipMemcpyAsync(&flags, J.flags, sizeof(int), hipMemcpyDeviceToHost, s));
+    UPPE_HIP(hipStreamSynchronize(s));
+    UPPE_CHECK(!(flags & 2), "uppe1d_adjoint: instance does not fit the launch configuration");
+    UPPE_CHECK(!(flags & 1),
+               "uppe1d_adjoint: plasma outside the differentiable regime (avalanche rate or a "
+               "density clamp is active)");
+    return UPPE_OK;
+}
 
 }  // namespace
 
@@ -816,6 +1331,19 @@ int uppe1d_batch_launch(void *workspace, int n, int first, int count, int thread
 
 int uppe1d_batch_status(void *workspace, int n, void *stream, uppe1d_status *st) {
     return status(workspace, n, (hipStream_t)stream, st);
+}
+
+int uppe1d_adjoint_workspace_size(int nt, int ckpt_every, size_t *bytes) {
+    UPPE_CHECK(bytes && nt > 0 && ckpt_every > 0, "uppe1d_adjoint_workspace_size: bad argument");
+    *bytes = adj_bytes(nt, ckpt_every);
+    return UPPE_OK;
+}
+
+int uppe1d_adjoint(void *fwd_workspace, int n, int index, int M, const double *hs,
+                   const void *ckpt, int ckpt_every, void *lam, void *adj_workspace, double *gD,
+                   double *gU, double *gG, void *stream) {
+    return adjoint(fwd_workspace, n, index, M, hs, ckpt, ckpt_every, lam, adj_workspace, gD, gU,
+                   gG, (hipStream_t)stream);
 }
 
 int uppe1d_propagate(const uppe1d_desc *d, const double *D_re, const double *D_im,
