@@ -116,6 +116,47 @@ def bench_batched(n=256, L=1.0):
             "nt_counts": {str(k): int((nts == k).sum()) for k in sorted(set(nts))}}
 
 
+def bench_adjoint():
+    """Config 3: gradient of ||S_out - S_target||^2 for the config-0 setup (N_t = 4096, 1 m):
+    one recorded forward solve and one backward pass, timed separately."""
+    import torch
+    from project_amd import workloads as wl
+    from project_amd.adjoint import (FrozenPropagator1D, UPPEPropagate1D, spectral_loss,
+                                     super_gaussian_target)
+    t, a0 = wl.pulse1d(4096)
+    omega = 2 * np.pi * np.fft.fftfreq(4096, t[1] - t[0])
+    base = np.fft.fft(a0)
+    dev = torch.device("cuda")
+    S_t = torch.from_numpy(super_gaussian_target(omega, base)).to(dev)
+    prop = FrozenPropagator1D(wl.argon_solver_1d(1.0), t, 1.0, tol=1e-6)
+    D0, U0, g0 = prop.tables()
+    A0w = torch.from_numpy(base).to(dev).to(torch.complex64)
+    prop.run(A0w, D0, U0, g0)          # adaptive run, freezes the step sequence
+    fw, bw = [], []
+    for _ in range(3):
+        phi = torch.zeros(4096, dtype=torch.float64, device=dev, requires_grad=True)
+        A = (torch.from_numpy(base).to(dev) * torch.exp(1j * phi)).to(torch.complex64)
+        D = torch.from_numpy(D0).to(dev)
+        U = torch.from_numpy(U0).to(dev)
+        G = torch.tensor(g0, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = UPPEPropagate1D.apply(A, D, U, G, prop)
+        loss = spectral_loss(out.to(torch.complex128), S_t)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        fw.append(t1 - t0)
+        bw.append(t2 - t1)
+    f, b = min(fw), min(bw)
+    return {"workload": "adjoint 1D config-0 setup: dL/d(phase[4096], energy, pressure), "
+                        "%d frozen steps, checkpoints every 16" % prop.M,
+            "forward_s": round(f, 5), "backward_s": round(b, 5),
+            "adjoint_over_forward": round(b / f, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -225,6 +266,7 @@ def main():
     if not args.no_extra:
         out["config0_1d"] = bench_config0(cpu=args.cpu_attempts > 0)
         out["batched_1d"] = bench_batched()
+        out["adjoint_1d"] = bench_adjoint()
     print(json.dumps(out), flush=True)
 
 
