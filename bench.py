@@ -168,6 +168,9 @@ def main():
                     help="oracle attempts timed for the CPU baseline (0 skips it)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the 1D, batched and adjoint legs")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="before the run, copy a MIB-MiB float32 buffer 3 times (a plain copy "
+                         "kernel of known traffic, to scale HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the 128x128x1024 field is not "
@@ -179,6 +182,13 @@ def main():
     from project_amd.solver_3d import ERK43IP_UPPE_3D_Optimized
 
     lib = _lib.load()
+    if args.copy_calibration > 0:
+        src = torch.ones(args.copy_calibration << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     nx, nt = args.nx, args.nt
     x, y, t = wl.grid3d(nx, nt)
     A0 = wl.beam3d(x, y, t)

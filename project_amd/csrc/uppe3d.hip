@@ -62,6 +62,7 @@ struct Ctl3D {
     int replay, replay_acc;
     unsigned int maxA_bits, maxD_bits;
     int cur, done, nonfinite, fail_count, attempts, accepted, rejected, pad_;
+    int plasma_groups[8];   // per t-pass slot of the attempt: workgroups deferred to plasma
 };
 
 struct Tabs3D {
@@ -203,11 +204,21 @@ __device__ __forceinline__ void lanes_dif(float2 (&x)[R], int lane, const float2
     }
 }
 
+// Returns p unchanged, hidden from the optimiser: loads through the result are not merged with
+// loads through p issued earlier, so twiddles are re-read from cache at each transform instead
+// of being held in registers across the whole kernel.
+template <class T>
+__device__ __forceinline__ const T *opaque(const T *p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
 // x[r] = X(omega = l + 64 r), l = bitrev6(lane)  ->  x[k] = A(t = k + R lane), scaled by 1/NT
 template <int R>
 __device__ __forceinline__ void wave_ifft_t(float2 (&x)[R], int lane, int rl,
-                                            const float2 *__restrict__ tw) {
+                                            const float2 *__restrict__ tw0) {
     constexpr int NT = 64 * R;
+    const float2 *tw = opaque(tw0);
     Dft<R, -1>::run(x);
 #pragma unroll
     for (int k = 1; k < R; ++k) x[k] = cmulc(x[k], tw[rl * k]);
@@ -220,8 +231,9 @@ __device__ __forceinline__ void wave_ifft_t(float2 (&x)[R], int lane, int rl,
 // x[k] = y(t = k + R lane)  ->  x[r] = Y(omega = l + 64 r), l = bitrev6(lane), unnormalised
 template <int R>
 __device__ __forceinline__ void wave_fft_t(float2 (&x)[R], int lane, int rl,
-                                           const float2 *__restrict__ tw) {
+                                           const float2 *__restrict__ tw0) {
     constexpr int NT = 64 * R;
+    const float2 *tw = opaque(tw0);
     lanes_dif<1, R>(x, lane, tw, NT);
 #pragma unroll
     for (int k = 1; k < R; ++k) x[k] = cmul(x[k], tw[rl * k]);
@@ -375,29 +387,36 @@ __device__ __forceinline__ bool plasma_matters(const float2 (&x)[R], const Phys3
 // B' = X * filter(omega) * exp(-damp(x,y) h) is written to the spare state buffer on the way.
 //
 // Two instantiations per (NT, MODE): PLASMA = false has no plasma code (so the FFTs keep their
-// registers); a workgroup whose lines need the plasma term records that in flags[] and
-// returns before storing.  PLASMA = true then runs only the flagged workgroups.
+// registers); a workgroup whose lines need the plasma term appends its index to a work list
+// and returns before storing.  PLASMA = true then runs only the listed workgroups.
+template <int NT, int LINES>
+struct TStage {
+    static constexpr int R = NT / 64;
+    static constexpr int CH = R < 4 ? R : 4;
+    static constexpr int SP = 66;   // padded stage row (float2): q-stride of 33 dword pairs
+    static constexpr int SIZE = CH * LINES * SP;
+};
+
 template <int NT, int LINES, int MODE, bool PLASMA>
-__global__ __launch_bounds__(64 * LINES) void tpass_kernel(Fields3D f, const float2 *in_fixed,
-                                                           int in_is_state, float2 *out,
-                                                           Tabs3D tb, Phys3D ph, Ctl3D *ctl,
-                                                           int nx, int ny, int *flags) {
-    constexpr int R = NT / 64;
-    constexpr int CH = R < 4 ? R : 4;
-    constexpr int SP = 66;   // padded stage row (float2): q-stride of 33 dword pairs
-    __shared__ float2 stage[CH * LINES * SP];
-    if (ctl->done) return;
-    if (PLASMA && flags[blockIdx.x] == 0) return;
+__device__ __forceinline__ void tpass_group(int blk, float2 *stage, Fields3D f,
+                                            const float2 *in_fixed, int in_is_state,
+                                            float2 *out, const Tabs3D &tb, const Phys3D &ph,
+                                            Ctl3D *ctl, int nx, int ny, int *list, int slot) {
+    constexpr int R = TStage<NT, LINES>::R;
+    constexpr int CH = TStage<NT, LINES>::CH;
+    constexpr int SP = TStage<NT, LINES>::SP;
     const int cur = ctl->cur;
     const float2 *in = in_is_state ? (cur ? f.B1 : f.B0) : in_fixed;
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int rl = __brev(lane) >> 26;
     const int bpx = ny / LINES;
-    const int x = blockIdx.x / bpx;
-    const int y0 = (blockIdx.x % bpx) * LINES;
-    const size_t plane = (size_t)nx * ny;
-    const size_t col0 = (size_t)x * ny + y0;
+    const int x = blk / bpx;
+    const int y0 = (blk % bpx) * LINES;
+    const int plane = nx * ny;
     const int q = tid % LINES, row = tid / LINES;
+    // 32-bit element offsets from a uniform base (fields hold < 2^28 elements)
+    const int col0 = x * ny + y0;
+    const int lo = row * plane + col0 + q;
 
     float2 v[R];
     {
@@ -411,19 +430,19 @@ __global__ __launch_bounds__(64 * LINES) void tpass_kernel(Fields3D f, const flo
         }
         float2 g[CH], gn[CH];
 #pragma unroll
-        for (int cc = 0; cc < CH; ++cc) g[cc] = in[(size_t)(row + 64 * cc) * plane + col0 + q];
+        for (int cc = 0; cc < CH; ++cc) g[cc] = in[lo + 64 * cc * plane];
 #pragma unroll
         for (int c0 = 0; c0 < R; c0 += CH) {
             if (c0 + CH < R) {
 #pragma unroll
                 for (int cc = 0; cc < CH; ++cc)
-                    gn[cc] = in[(size_t)(row + 64 * (c0 + CH + cc)) * plane + col0 + q];
+                    gn[cc] = in[lo + 64 * (c0 + CH + cc) * plane];
             }
             if (MODE == 1) {
 #pragma unroll
                 for (int cc = 0; cc < CH; ++cc) {
                     const int w = row + 64 * (c0 + cc);
-                    bnew[(size_t)w * plane + col0 + q] = cscale(g[cc], tb.filt_w[w] * absorb);
+                    bnew[lo + 64 * (c0 + cc) * plane] = cscale(g[cc], tb.filt_w[w] * absorb);
                 }
             }
 #pragma unroll
@@ -441,8 +460,10 @@ __global__ __launch_bounds__(64 * LINES) void tpass_kernel(Fields3D f, const flo
     bool plasma = plasma_matters<R>(v, ph, tb.lut, (double)ctl->h);
     if (!PLASMA) {
         const int defer = __syncthreads_or(plasma ? 1 : 0);
-        if (tid == 0) flags[blockIdx.x] = defer;
-        if (defer) return;
+        if (defer) {
+            if (tid == 0) list[atomicAdd(&ctl->plasma_groups[slot], 1)] = blk;
+            return;
+        }
         plasma = false;
     }
 
@@ -497,12 +518,34 @@ __global__ __launch_bounds__(64 * LINES) void tpass_kernel(Fields3D f, const flo
         __syncthreads();
 #pragma unroll
         for (int cc = 0; cc < CH; ++cc) {
-            const size_t g = (size_t)(row + 64 * (c0 + cc)) * plane + col0 + q;
+            const int g = lo + 64 * (c0 + cc) * plane;
             float2 val = stage[(cc * LINES + q) * SP + row];
             if (MODE == 1) val = csub(f.Y4[g], val);
             out[g] = val;
         }
         __syncthreads();
+    }
+}
+
+// t-pass launch pair.  PLASMA = false runs every workgroup of lines; PLASMA = true runs the
+// workgroups the first launch deferred (ctl->plasma_groups[slot] entries of list), a fixed
+// grid looping over them.  The counters are cleared by the controller at the end of each
+// attempt and by begin_kernel.
+template <int NT, int LINES, int MODE, bool PLASMA>
+__global__ __launch_bounds__(64 * LINES) void tpass_kernel(Fields3D f, const float2 *in_fixed,
+                                                           int in_is_state, float2 *out,
+                                                           Tabs3D tb, Phys3D ph, Ctl3D *ctl,
+                                                           int nx, int ny, int *list, int slot) {
+    __shared__ float2 stage[TStage<NT, LINES>::SIZE];
+    if (ctl->done) return;
+    if (!PLASMA) {
+        tpass_group<NT, LINES, MODE, false>(blockIdx.x, stage, f, in_fixed, in_is_state, out, tb,
+                                            ph, ctl, nx, ny, list, slot);
+    } else {
+        const int n = ctl->plasma_groups[slot];
+        for (int i = blockIdx.x; i < n; i += gridDim.x)
+            tpass_group<NT, LINES, MODE, true>(list[i], stage, f, in_fixed, in_is_state, out,
+                                               tb, ph, ctl, nx, ny, list, slot);
     }
 }
 
@@ -663,27 +706,20 @@ struct ColFFT<NX, NY, T, NX, DIR> {
     __device__ __forceinline__ static void run(float2 *, int, const float2 *) {}
 };
 
-// 2D FFT of a plane held in registers (element e = tid + T*i, natural x-major order).
-// Forward unnormalised; inverse scaled by 1/(NX*NY).
-template <int NX, int NY, int DIR, int E>
-__device__ __forceinline__ void fft2(float2 (&a)[E], float2 *lds, int tid,
-                                     const Tabs3D &tb) {
+// 2D FFT of the plane held in LDS (rows padded to LD), in place.  Forward unnormalised;
+// the caller applies 1/(NX*NY) after an inverse.  The plane must be complete in LDS (a
+// barrier after the writes); the transform ends with a barrier.
+template <int NX, int NY, int DIR>
+__device__ __forceinline__ void fft2_lds(float2 *lds, int tid, const Tabs3D &tb) {
     using C = XY<NX, NY>;
-#pragma unroll
-    for (int i = 0; i < C::EPT; ++i) {
-        const int e = tid + C::T * i;
-        lds[(e / NY) * C::LD + (e % NY)] = a[i];
-    }
-    __syncthreads();
-    RowFFT<NX, NY, C::T, 1, DIR>::run(lds, tid, tb.tw_y);
-    ColFFT<NX, NY, C::T, 1, DIR>::run(lds, tid, tb.tw_x);
-    const float s = DIR > 0 ? 1.0f : 1.0f / (float)(NX * NY);
-#pragma unroll
-    for (int i = 0; i < C::EPT; ++i) {
-        const int e = tid + C::T * i;
-        a[i] = cscale(lds[(e / NY) * C::LD + (e % NY)], s);
-    }
-    __syncthreads();
+    RowFFT<NX, NY, C::T, 1, DIR>::run(lds, tid, opaque(tb.tw_y));
+    ColFFT<NX, NY, C::T, 1, DIR>::run(lds, tid, opaque(tb.tw_x));
+}
+
+// LDS slot of plane element e (x-major, e = x * NY + y)
+template <int NY>
+__device__ __forceinline__ int lslot(int e) {
+    return (e / NY) * (NY + 1) + (e % NY);
 }
 
 // exp(s * D(kx, ky, omega_w)) with D = i (kz - k_ref), kz the principal sqrt of k^2 - kperp^2,
@@ -726,6 +762,13 @@ __device__ __forceinline__ void block_max_to(unsigned int m, unsigned int *dst, 
     }
 }
 
+// MODE 2: X = iFFT2(E (FFT2(B) + h/2 FFT2(Y))), V = E FFT2(B), Q = E (FFT2(B) + h/6 FFT2(Y))
+// MODE 3: X = iFFT2(V + h/2 FFT2(Y)),            Q += h/3 FFT2(Y)
+// MODE 4: X = iFFT2(E (V + h FFT2(Y))),           Q = E (Q + h/3 FFT2(Y))
+// MODE 5: A4 = Q + h/6 FFT2(Y4), max |A4|^2,      X = iFFT2(A4)
+// MODE 6: max |FFT2(Y)|^2 (Y = Y4 - Y5 from the last t-pass)
+// Transformed planes stay in LDS; only FFT2(Y) of MODE 2 is held in registers while the
+// state plane is transformed.
 template <int NX, int NY, int MODE>
 __global__ __launch_bounds__(xy_threads(NX, NY)) void xypass_kernel(Fields3D f, Tabs3D tb,
                                                                Ctl3D *ctl) {
@@ -737,75 +780,83 @@ __global__ __launch_bounds__(xy_threads(NX, NY)) void xypass_kernel(Fields3D f, 
     const int tid = threadIdx.x;
     const size_t pb = (size_t)w * C::P;
     const float h = (float)ctl->h;
-    float2 a[C::EPT], b[C::EPT];
-    if (MODE == 2) {
-        const float2 *B = ctl->cur ? f.B1 : f.B0;
+    const float inv = 1.0f / (float)(NX * NY);
+    // per-plane base pointers (uniform) and 32-bit in-plane offsets
+    float2 *const Xp = f.X + pb, *const Vp = f.V + pb, *const Qp = f.Q + pb;
+    const float2 *const Yp = (MODE == 5 ? f.Y4 : f.Yw) + pb;
 #pragma unroll
-        for (int i = 0; i < C::EPT; ++i) {
-            a[i] = B[pb + tid + C::T * i];
-            b[i] = f.Yw[pb + tid + C::T * i];
-        }
-        fft2<NX, NY, 1>(a, lds, tid, tb);
-        fft2<NX, NY, 1>(b, lds, tid, tb);
-#pragma unroll
-        for (int i = 0; i < C::EPT; ++i) {
-            const int e = tid + C::T * i;
-            const float2 E = eprop<NY>(e, w, 0.5f * h, tb);
-            f.V[pb + e] = cmul(E, a[i]);
-            f.Q[pb + e] = cmul(E, caxpy(a[i], h / 6.0f, b[i]));
-            a[i] = cmul(E, caxpy(a[i], 0.5f * h, b[i]));
-        }
-        fft2<NX, NY, -1>(a, lds, tid, tb);
-#pragma unroll
-        for (int i = 0; i < C::EPT; ++i) f.X[pb + tid + C::T * i] = a[i];
-    } else if (MODE == 3 || MODE == 4) {
-#pragma unroll
-        for (int i = 0; i < C::EPT; ++i) b[i] = f.Yw[pb + tid + C::T * i];
-        fft2<NX, NY, 1>(b, lds, tid, tb);
-#pragma unroll
-        for (int i = 0; i < C::EPT; ++i) {
-            const int e = tid + C::T * i;
-            const float2 v = f.V[pb + e];
-            const float2 qv = f.Q[pb + e];
-            if (MODE == 3) {
-                a[i] = caxpy(v, 0.5f * h, b[i]);
-                f.Q[pb + e] = caxpy(qv, h / 3.0f, b[i]);
-            } else {
-                const float2 E = eprop<NY>(e, w, 0.5f * h, tb);
-                a[i] = cmul(E, caxpy(v, h, b[i]));
-                f.Q[pb + e] = cmul(E, caxpy(qv, h / 3.0f, b[i]));
-            }
-        }
-        fft2<NX, NY, -1>(a, lds, tid, tb);
-#pragma unroll
-        for (int i = 0; i < C::EPT; ++i) f.X[pb + tid + C::T * i] = a[i];
-    } else if (MODE == 5) {
-#pragma unroll
-        for (int i = 0; i < C::EPT; ++i) b[i] = f.Y4[pb + tid + C::T * i];
-        fft2<NX, NY, 1>(b, lds, tid, tb);
+    for (int i = 0; i < C::EPT; ++i) {
+        const int e = tid + C::T * i;
+        lds[lslot<NY>(e)] = Yp[e];
+    }
+    __syncthreads();
+    fft2_lds<NX, NY, 1>(lds, tid, tb);
+    if (MODE == 6) {
         unsigned int m = 0;
 #pragma unroll
         for (int i = 0; i < C::EPT; ++i) {
-            const int e = tid + C::T * i;
-            a[i] = caxpy(f.Q[pb + e], h / 6.0f, b[i]);
-            const unsigned int v = __float_as_uint(cabs2(a[i]));
-            m = v > m ? v : m;
-        }
-        block_max_to<C::T>(m, &ctl->maxA_bits, red, tid);
-        fft2<NX, NY, -1>(a, lds, tid, tb);
-#pragma unroll
-        for (int i = 0; i < C::EPT; ++i) f.X[pb + tid + C::T * i] = a[i];
-    } else {  // MODE 6
-#pragma unroll
-        for (int i = 0; i < C::EPT; ++i) b[i] = f.Yw[pb + tid + C::T * i];
-        fft2<NX, NY, 1>(b, lds, tid, tb);
-        unsigned int m = 0;
-#pragma unroll
-        for (int i = 0; i < C::EPT; ++i) {
-            const unsigned int v = __float_as_uint(cabs2(b[i]));
+            const unsigned int v = __float_as_uint(cabs2(lds[lslot<NY>(tid + C::T * i)]));
             m = v > m ? v : m;
         }
         block_max_to<C::T>(m, &ctl->maxD_bits, red, tid);
+        return;
+    }
+    if (MODE == 2) {
+        float2 b[C::EPT];
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) b[i] = lds[lslot<NY>(tid + C::T * i)];
+        __syncthreads();
+        const float2 *B = (ctl->cur ? f.B1 : f.B0) + pb;
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) {
+            const int e = tid + C::T * i;
+            lds[lslot<NY>(e)] = B[e];
+        }
+        __syncthreads();
+        fft2_lds<NX, NY, 1>(lds, tid, tb);
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) {
+            const int e = tid + C::T * i;
+            const float2 a = lds[lslot<NY>(e)];
+            const float2 E = eprop<NY>(e, w, 0.5f * h, tb);
+            Vp[e] = cmul(E, a);
+            Qp[e] = cmul(E, caxpy(a, h / 6.0f, b[i]));
+            lds[lslot<NY>(e)] = cmul(E, caxpy(a, 0.5f * h, b[i]));
+        }
+    } else if (MODE == 3 || MODE == 4) {
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) {
+            const int e = tid + C::T * i;
+            const float2 bh = lds[lslot<NY>(e)];
+            const float2 v = Vp[e];
+            const float2 qv = Qp[e];
+            if (MODE == 3) {
+                lds[lslot<NY>(e)] = caxpy(v, 0.5f * h, bh);
+                Qp[e] = caxpy(qv, h / 3.0f, bh);
+            } else {
+                const float2 E = eprop<NY>(e, w, 0.5f * h, tb);
+                lds[lslot<NY>(e)] = cmul(E, caxpy(v, h, bh));
+                Qp[e] = cmul(E, caxpy(qv, h / 3.0f, bh));
+            }
+        }
+    } else {  // MODE 5
+        unsigned int m = 0;
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) {
+            const int e = tid + C::T * i;
+            const float2 a = caxpy(Qp[e], h / 6.0f, lds[lslot<NY>(e)]);
+            const unsigned int v = __float_as_uint(cabs2(a));
+            m = v > m ? v : m;
+            lds[lslot<NY>(e)] = a;
+        }
+        block_max_to<C::T>(m, &ctl->maxA_bits, red, tid);
+    }
+    __syncthreads();
+    fft2_lds<NX, NY, -1>(lds, tid, tb);
+#pragma unroll
+    for (int i = 0; i < C::EPT; ++i) {
+        const int e = tid + C::T * i;
+        Xp[e] = cscale(lds[lslot<NY>(e)], inv);
     }
 }
 
@@ -813,6 +864,7 @@ __global__ __launch_bounds__(xy_threads(NX, NY)) void xypass_kernel(Fields3D f, 
 // Step controller (solver_3d.py:182-208), one thread, FP64.  Also applies the top-of-loop
 // clip h = min(h, L - z) of the next iteration and the h < 1e-15 stop.
 __global__ void ctl_kernel(Ctl3D *c) {
+    for (int i = 0; i < 8; ++i) c->plasma_groups[i] = 0;
     if (c->done) return;
     const double h = c->h;
     const unsigned int ab = c->maxA_bits, db = c->maxD_bits;
@@ -864,6 +916,7 @@ __global__ void ctl_kernel(Ctl3D *c) {
 }
 
 __global__ void begin_kernel(Ctl3D *c, double L, double tol, double max_step, double min_step) {
+    for (int i = 0; i < 8; ++i) c->plasma_groups[i] = 0;
     c->z = 0.0;
     c->err = 0.0;
     c->replay_h = 0.0;
@@ -905,7 +958,8 @@ struct uppe3d_plan {
     Tabs3D tb;
     Phys3D ph;
     Ctl3D *ctl;
-    int *flags;   // per t-pass workgroup: its lines need the plasma term
+    int *list;    // t-pass workgroups deferred to the plasma instantiation
+    int ncu;      // compute units of the device (grid of the plasma launch)
     size_t ws_bytes;
 };
 
@@ -916,7 +970,7 @@ size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 struct Layout3D {
     size_t tw_t, tw_x, tw_y, d_w, k_w, k2_w, kref_w, valid_w, filt_w, shock_w, mask_w, kx2, ky2,
-        damp, lut, ctl, flags, fields[7], total;
+        damp, lut, ctl, list, fields[7], total;
 };
 
 Layout3D layout3d(const uppe3d_desc &d) {
@@ -943,7 +997,7 @@ Layout3D layout3d(const uppe3d_desc &d) {
     L.damp = take(4 * (size_t)d.nx * d.ny);
     L.lut = take(8 * (size_t)d.lut_n);
     L.ctl = take(sizeof(Ctl3D));
-    L.flags = take(sizeof(int) * (size_t)d.nx * d.ny);
+    L.list = take(sizeof(int) * (size_t)d.nx * d.ny);
     const size_t field = sizeof(float2) * (size_t)d.nx * d.ny * d.nt;
     for (int i = 0; i < 7; ++i) L.fields[i] = take(field);
     L.total = o;
@@ -976,22 +1030,24 @@ void twiddles(std::vector<float2> &v, int n) {
     case NT:                                                                                 \
         if (mode == 0) {                                                                     \
             tpass_kernel<NT, LINES, 0, false><<<grid, 64 * LINES, 0, s>>>(                   \
-                p->f, in, in_is_state, out, p->tb, p->ph, p->ctl, nx, ny, p->flags);         \
-            tpass_kernel<NT, LINES, 0, true><<<grid, 64 * LINES, 0, s>>>(                    \
-                p->f, in, in_is_state, out, p->tb, p->ph, p->ctl, nx, ny, p->flags);         \
+                p->f, in, in_is_state, out, p->tb, p->ph, p->ctl, nx, ny, p->list, slot);    \
+            tpass_kernel<NT, LINES, 0, true><<<pgrid, 64 * LINES, 0, s>>>(                   \
+                p->f, in, in_is_state, out, p->tb, p->ph, p->ctl, nx, ny, p->list, slot);    \
         } else {                                                                             \
             tpass_kernel<NT, LINES, 1, false><<<grid, 64 * LINES, 0, s>>>(                   \
-                p->f, in, in_is_state, out, p->tb, p->ph, p->ctl, nx, ny, p->flags);         \
-            tpass_kernel<NT, LINES, 1, true><<<grid, 64 * LINES, 0, s>>>(                    \
-                p->f, in, in_is_state, out, p->tb, p->ph, p->ctl, nx, ny, p->flags);         \
+                p->f, in, in_is_state, out, p->tb, p->ph, p->ctl, nx, ny, p->list, slot);    \
+            tpass_kernel<NT, LINES, 1, true><<<pgrid, 64 * LINES, 0, s>>>(                   \
+                p->f, in, in_is_state, out, p->tb, p->ph, p->ctl, nx, ny, p->list, slot);    \
         }                                                                                    \
         break;
 
-int launch_tpass(uppe3d_plan *p, int mode, const float2 *in, int in_is_state, float2 *out,
-                 hipStream_t s) {
+// slot: index (0..4) of this t-pass within the attempt (its plasma work-list counter)
+int launch_tpass(uppe3d_plan *p, int mode, int slot, const float2 *in, int in_is_state,
+                 float2 *out, hipStream_t s) {
     const int nx = p->d.nx, ny = p->d.ny;
     const int lines = p->d.nt >= 2048 ? 8 : 16;
-    const dim3 grid(nx * (ny / lines));
+    const int groups = nx * (ny / lines);
+    const dim3 grid(groups), pgrid(groups < p->ncu ? groups : p->ncu);
     switch (p->d.nt) {
         LAUNCH_T(128, 16)
         LAUNCH_T(256, 16)
@@ -1045,23 +1101,23 @@ int enqueue_attempt(uppe3d_plan *p, hipStream_t s, hipEvent_t *ev = nullptr) {
         return UPPE_OK;
     };
     RET(mark());
-    RET(launch_tpass(p, 0, nullptr, 1, f.Yw, s));   // T1 on the state
+    RET(launch_tpass(p, 0, 0, nullptr, 1, f.Yw, s));   // T1 on the state
     RET(mark());
     RET(launch_xy(p, 2, s));                          // P2
     RET(mark());
-    RET(launch_tpass(p, 0, f.X, 0, f.Yw, s));        // T2
+    RET(launch_tpass(p, 0, 1, f.X, 0, f.Yw, s));        // T2
     RET(mark());
     RET(launch_xy(p, 3, s));                          // P3
     RET(mark());
-    RET(launch_tpass(p, 0, f.X, 0, f.Yw, s));        // T3
+    RET(launch_tpass(p, 0, 2, f.X, 0, f.Yw, s));        // T3
     RET(mark());
     RET(launch_xy(p, 4, s));                          // P4
     RET(mark());
-    RET(launch_tpass(p, 0, f.X, 0, f.Y4, s));        // T4
+    RET(launch_tpass(p, 0, 3, f.X, 0, f.Y4, s));        // T4
     RET(mark());
     RET(launch_xy(p, 5, s));                          // P5
     RET(mark());
-    RET(launch_tpass(p, 1, f.X, 0, f.Yw, s));        // T5 (D and next state)
+    RET(launch_tpass(p, 1, 4, f.X, 0, f.Yw, s));        // T5 (D and next state)
     RET(mark());
     RET(launch_xy(p, 6, s));                          // P6
     RET(mark());
@@ -1143,7 +1199,13 @@ int uppe3d_plan_create(const uppe3d_desc *d, void *workspace, size_t workspace_b
     p->d = *d;
     p->ws_bytes = L.total;
     p->ctl = (Ctl3D *)(ws + L.ctl);
-    p->flags = (int *)(ws + L.flags);
+    p->list = (int *)(ws + L.list);
+    {
+        int dev = 0;
+        UPPE_HIP(hipGetDevice(&dev));
+        UPPE_HIP(hipDeviceGetAttribute(&p->ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        if (p->ncu < 1) p->ncu = 1;
+    }
     float2 **fp[7] = {&p->f.B0, &p->f.B1, &p->f.X, &p->f.Yw, &p->f.Y4, &p->f.V, &p->f.Q};
     for (int i = 0; i < 7; ++i) *fp[i] = (float2 *)(ws + L.fields[i]);
     Tabs3D &tb = p->tb;
