@@ -67,6 +67,7 @@ struct Ctl3D {
 
 struct Tabs3D {
     const float2 *tw_t, *tw_x, *tw_y;
+    const float2 *twA, *twB;   // lane twiddles of the NT = 1024 line transform (tw1024_*)
     const float *d_w, *k_w, *k2_w, *kref_w, *valid_w, *filt_w, *shock_w, *mask_w;
     const float *kx2, *ky2, *damp;
     const double *lut;
@@ -240,6 +241,92 @@ __device__ __forceinline__ void wave_fft_t(float2 (&x)[R], int lane, int rl,
     Dft<R, 1>::run(x);
 }
 
+// ---------------------------------------------------------------------------------------
+// NT = 1024 line transform held by one wavefront, 16 values per lane, as three register DFT
+// passes (16 x 16 x 4) with two transposes through a wave-private LDS region:
+//   omega = lam + 64 r,  lam = la + 4 lb,  t = t1 + 16 u + 256 v
+//   inverse: DFT16 over r -> x W1024^(-lam t1) -> [LDS] -> DFT16 over lb -> x W64^(-la u)
+//            -> [LDS] -> DFT4 over la.
+// The forward transform is the transpose of that sequence (DFT4, LDS, twiddle, DFT16, LDS,
+// twiddle, DFT16) with the conjugate twiddles.  Lane twiddle tables (row-major, 64 lanes per
+// row): twA[t1 * 64 + lam] = W1024^(lam t1), twB[u * 64 + mu] = W64^((mu & 3) u).
+// Frequency layout: v[r] = X[lane + 64 r].  Time layout: with t1 = lane & 15, ug = lane >> 4,
+// v[4 j + vv] = x[t1 + 16 (4 ug + j) + 256 vv] (any pointwise operation may use it).
+struct W1024 {
+    static constexpr int P1 = 68;        // first transpose: row pitch (per t1)
+    static constexpr int S1 = 260;       // second transpose: pitch per la
+    static constexpr int REGION = 16 * P1;  // float2 per wavefront (>= 3 S1 + 256)
+};
+
+__device__ __forceinline__ void w1024_inverse(float2 (&v)[16], float2 *buf, int lane,
+                                              const float2 *__restrict__ twA0,
+                                              const float2 *__restrict__ twB0) {
+    Dft<16, -1>::run(v);
+    const float2 *twA = opaque(twA0);
+#pragma unroll
+    for (int k = 1; k < 16; ++k) v[k] = cmulc(v[k], twA[k * 64 + lane]);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) buf[k * W1024::P1 + lane] = v[k];
+    wave_lds_sync();
+    const int t1p = lane >> 2, la = lane & 3;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = buf[t1p * W1024::P1 + la + 4 * k];
+    wave_lds_sync();
+    Dft<16, -1>::run(v);
+    const float2 *twB = opaque(twB0);
+#pragma unroll
+    for (int k = 1; k < 16; ++k) v[k] = cmulc(v[k], twB[k * 64 + lane]);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) buf[la * W1024::S1 + k * 16 + t1p] = v[k];
+    wave_lds_sync();
+    const int t1 = lane & 15, ug = lane >> 4;
+    constexpr float s = 1.0f / 1024.0f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        float2 w[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) w[a] = buf[a * W1024::S1 + (4 * ug + j) * 16 + t1];
+        Dft<4, -1>::run(w);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) v[4 * j + b] = cscale(w[b], s);
+    }
+    wave_lds_sync();
+}
+
+__device__ __forceinline__ void w1024_forward(float2 (&v)[16], float2 *buf, int lane,
+                                              const float2 *__restrict__ twA0,
+                                              const float2 *__restrict__ twB0) {
+    const int t1 = lane & 15, ug = lane >> 4;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        float2 w[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) w[b] = v[4 * j + b];
+        Dft<4, 1>::run(w);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) buf[a * W1024::S1 + (4 * ug + j) * 16 + t1] = w[a];
+    }
+    wave_lds_sync();
+    const int t1p = lane >> 2, la = lane & 3;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = buf[la * W1024::S1 + k * 16 + t1p];
+    wave_lds_sync();
+    const float2 *twB = opaque(twB0);
+#pragma unroll
+    for (int k = 1; k < 16; ++k) v[k] = cmul(v[k], twB[k * 64 + lane]);
+    Dft<16, 1>::run(v);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) buf[t1p * W1024::P1 + la + 4 * k] = v[k];
+    wave_lds_sync();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = buf[k * W1024::P1 + lane];
+    wave_lds_sync();
+    const float2 *twA = opaque(twA0);
+#pragma unroll
+    for (int k = 1; k < 16; ++k) v[k] = cmul(v[k], twA[k * 64 + lane]);
+    Dft<16, 1>::run(v);
+}
+
 // One step of the reference recurrence (media_physics.py:36-63) from rho; returns the
 // plasma coefficients (c1, c2) of the sample and advances rho.  k1_affine: avalanche rate
 // zero, so Gamma = -1/tau_rec and exp(gdt), the step factor are constants.
@@ -389,12 +476,29 @@ __device__ __forceinline__ bool plasma_matters(const float2 (&x)[R], const Phys3
 // Two instantiations per (NT, MODE): PLASMA = false has no plasma code (so the FFTs keep their
 // registers); a workgroup whose lines need the plasma term appends its index to a work list
 // and returns before storing.  PLASMA = true then runs only the listed workgroups.
+// Forward line transform in the layout of the instantiation (see W1024 and wave_fft_t).
+template <int R, bool FAST>
+__device__ __forceinline__ void line_forward(float2 (&v)[R], float2 *buf, int lane, int rl,
+                                             const Tabs3D &tb) {
+    if constexpr (FAST) {
+        w1024_forward(*reinterpret_cast<float2(*)[16]>(&v[0]), buf, lane, tb.twA, tb.twB);
+    } else {
+        wave_fft_t<R>(v, lane, rl, tb.tw_t);
+    }
+}
+
 template <int NT, int LINES>
 struct TStage {
     static constexpr int R = NT / 64;
     static constexpr int CH = R < 4 ? R : 4;
     static constexpr int SP = 66;   // padded stage row (float2): q-stride of 33 dword pairs
     static constexpr int SIZE = CH * LINES * SP;
+    // lean NT = 1024 instantiation: the line transforms run through LDS (W1024), in a region
+    // shared with the HBM stage (separated by workgroup barriers)
+    static constexpr bool fast(bool plasma) { return NT == 1024 && !plasma; }
+    static constexpr int lds(bool plasma) {
+        return fast(plasma) && LINES * W1024::REGION > SIZE ? LINES * W1024::REGION : SIZE;
+    }
 };
 
 template <int NT, int LINES, int MODE, bool PLASMA>
@@ -405,10 +509,13 @@ __device__ __forceinline__ void tpass_group(int blk, float2 *stage, Fields3D f,
     constexpr int R = TStage<NT, LINES>::R;
     constexpr int CH = TStage<NT, LINES>::CH;
     constexpr int SP = TStage<NT, LINES>::SP;
+    constexpr bool FAST = TStage<NT, LINES>::fast(PLASMA);
     const int cur = ctl->cur;
     const float2 *in = in_is_state ? (cur ? f.B1 : f.B0) : in_fixed;
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int rl = __brev(lane) >> 26;
+    // lane's first frequency in the register layout of the line transform
+    const int sl = FAST ? lane : rl;
     const int bpx = ny / LINES;
     const int x = blk / bpx;
     const int y0 = (blk % bpx) * LINES;
@@ -449,13 +556,18 @@ __device__ __forceinline__ void tpass_group(int blk, float2 *stage, Fields3D f,
             for (int cc = 0; cc < CH; ++cc) stage[(cc * LINES + q) * SP + row] = g[cc];
             __syncthreads();
 #pragma unroll
-            for (int cc = 0; cc < CH; ++cc) v[c0 + cc] = stage[(cc * LINES + wv) * SP + rl];
+            for (int cc = 0; cc < CH; ++cc) v[c0 + cc] = stage[(cc * LINES + wv) * SP + sl];
             __syncthreads();
 #pragma unroll
             for (int cc = 0; cc < CH; ++cc) g[cc] = gn[cc];
         }
     }
-    wave_ifft_t<R>(v, lane, rl, tb.tw_t);
+    if constexpr (FAST) {
+        w1024_inverse(*reinterpret_cast<float2(*)[16]>(&v[0]), stage + wv * W1024::REGION,
+                      lane, tb.twA, tb.twB);
+    } else {
+        wave_ifft_t<R>(v, lane, rl, tb.tw_t);
+    }
 
     bool plasma = plasma_matters<R>(v, ph, tb.lut, (double)ctl->h);
     if (!PLASMA) {
@@ -467,7 +579,23 @@ __device__ __forceinline__ void tpass_group(int blk, float2 *stage, Fields3D f,
         plasma = false;
     }
 
-    if (!ph.use_shock) {
+    if constexpr (!PLASMA) {
+        // N = mask [shock] FFT_t(Kerr): one code path, the shock operator as a factor
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const float I = ph.a_scale2 * cabs2(v[k]);
+            const float kap = I * (ph.g_n2 + ph.g_n4 * I);
+            v[k] = make_float2(-kap * v[k].y, kap * v[k].x);
+        }
+        line_forward<R, FAST>(v, stage + wv * W1024::REGION, lane, rl, tb);
+        const float *mask = opaque(tb.mask_w), *shock = opaque(tb.shock_w);
+        const bool use_shock = ph.use_shock != 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int w = sl + 64 * r;
+            v[r] = cscale(v[r], use_shock ? shock[w] * mask[w] : mask[w]);
+        }
+    } else if (!ph.use_shock) {
         // N = mask FFT_t(Kerr + Plasma), formed in place
         auto total = [&](int k, float c1, float c2) {
             const float I = ph.a_scale2 * cabs2(v[k]);
@@ -480,9 +608,10 @@ __device__ __forceinline__ void tpass_group(int blk, float2 *stage, Fields3D f,
 #pragma unroll
             for (int k = 0; k < R; ++k) total(k, 0.0f, 0.0f);
         }
-        wave_fft_t<R>(v, lane, rl, tb.tw_t);
+        line_forward<R, FAST>(v, stage + wv * W1024::REGION, lane, rl, tb);
+        const float *mask = opaque(tb.mask_w);   // loaded here, not held across the transform
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[r] = cscale(v[r], tb.mask_w[rl + 64 * r]);
+        for (int r = 0; r < R; ++r) v[r] = cscale(v[r], mask[sl + 64 * r]);
     } else {
         // N = mask (shock FFT_t(Kerr) + FFT_t(Plasma))
         float2 p[R];
@@ -498,23 +627,25 @@ __device__ __forceinline__ void tpass_group(int blk, float2 *stage, Fields3D f,
             const float kap = I * (ph.g_n2 + ph.g_n4 * I);
             v[k] = make_float2(-kap * v[k].y, kap * v[k].x);
         }
-        wave_fft_t<R>(v, lane, rl, tb.tw_t);
+        line_forward<R, FAST>(v, stage + wv * W1024::REGION, lane, rl, tb);
+        const float *mask = opaque(tb.mask_w), *shock = opaque(tb.shock_w);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const int w = rl + 64 * r;
-            v[r] = cscale(v[r], tb.shock_w[w] * tb.mask_w[w]);
+            const int w = sl + 64 * r;
+            v[r] = cscale(v[r], shock[w] * mask[w]);
         }
         if (PLASMA && plasma) {
             wave_fft_t<R>(p, lane, rl, tb.tw_t);
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[r] = caxpy(v[r], tb.mask_w[rl + 64 * r], p[r]);
+            for (int r = 0; r < R; ++r) v[r] = caxpy(v[r], tb.mask_w[sl + 64 * r], p[r]);
         }
     }
 
+    if constexpr (FAST) __syncthreads();   // every wave is done with its transform region
 #pragma unroll
     for (int c0 = 0; c0 < R; c0 += CH) {
 #pragma unroll
-        for (int cc = 0; cc < CH; ++cc) stage[(cc * LINES + wv) * SP + rl] = v[c0 + cc];
+        for (int cc = 0; cc < CH; ++cc) stage[(cc * LINES + wv) * SP + sl] = v[c0 + cc];
         __syncthreads();
 #pragma unroll
         for (int cc = 0; cc < CH; ++cc) {
@@ -536,7 +667,7 @@ __global__ __launch_bounds__(64 * LINES) void tpass_kernel(Fields3D f, const flo
                                                            int in_is_state, float2 *out,
                                                            Tabs3D tb, Phys3D ph, Ctl3D *ctl,
                                                            int nx, int ny, int *list, int slot) {
-    __shared__ float2 stage[TStage<NT, LINES>::SIZE];
+    __shared__ float2 stage[TStage<NT, LINES>::lds(PLASMA)];
     if (ctl->done) return;
     if (!PLASMA) {
         tpass_group<NT, LINES, MODE, false>(blockIdx.x, stage, f, in_fixed, in_is_state, out, tb,
@@ -969,7 +1100,7 @@ constexpr size_t kAlign = 256;
 size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 struct Layout3D {
-    size_t tw_t, tw_x, tw_y, d_w, k_w, k2_w, kref_w, valid_w, filt_w, shock_w, mask_w, kx2, ky2,
+    size_t tw_t, tw_x, tw_y, twA, twB, d_w, k_w, k2_w, kref_w, valid_w, filt_w, shock_w, mask_w, kx2, ky2,
         damp, lut, ctl, list, fields[7], total;
 };
 
@@ -982,6 +1113,8 @@ Layout3D layout3d(const uppe3d_desc &d) {
         return at;
     };
     L.tw_t = take(sizeof(float2) * d.nt);
+    L.twA = take(sizeof(float2) * 16 * 64);
+    L.twB = take(sizeof(float2) * 16 * 64);
     L.tw_x = take(sizeof(float2) * d.nx);
     L.tw_y = take(sizeof(float2) * d.ny);
     L.d_w = take(4 * d.nt);
@@ -1175,6 +1308,21 @@ int uppe3d_plan_create(const uppe3d_desc *d, void *workspace, size_t workspace_b
     for (size_t i = 0; i < (size_t)nx * ny; ++i) dm[i] = (float)damp[i];
     // copies are synchronous so the host vectors may go out of scope
     UPPE_HIP(hipMemcpy(ws + L.tw_t, twt.data(), sizeof(float2) * nt, hipMemcpyHostToDevice));
+    {
+        // lane twiddles of the NT = 1024 line transform (W1024): rows of 64 lanes
+        std::vector<float2> ta(16 * 64), tbl(16 * 64);
+        for (int k = 0; k < 16; ++k)
+            for (int l = 0; l < 64; ++l) {
+                const double a = -2.0 * M_PI * (double)(l * k) / 1024.0;
+                const double b = -2.0 * M_PI * (double)((l & 3) * k) / 64.0;
+                ta[k * 64 + l] = make_float2((float)cos(a), (float)sin(a));
+                tbl[k * 64 + l] = make_float2((float)cos(b), (float)sin(b));
+            }
+        UPPE_HIP(hipMemcpy(ws + L.twA, ta.data(), sizeof(float2) * ta.size(),
+                           hipMemcpyHostToDevice));
+        UPPE_HIP(hipMemcpy(ws + L.twB, tbl.data(), sizeof(float2) * tbl.size(),
+                           hipMemcpyHostToDevice));
+    }
     UPPE_HIP(hipMemcpy(ws + L.tw_x, twx.data(), sizeof(float2) * nx, hipMemcpyHostToDevice));
     UPPE_HIP(hipMemcpy(ws + L.tw_y, twy.data(), sizeof(float2) * ny, hipMemcpyHostToDevice));
     UPPE_HIP(hipMemcpy(ws + L.d_w, dw.data(), 4 * nt, hipMemcpyHostToDevice));
@@ -1210,6 +1358,8 @@ int uppe3d_plan_create(const uppe3d_desc *d, void *workspace, size_t workspace_b
     for (int i = 0; i < 7; ++i) *fp[i] = (float2 *)(ws + L.fields[i]);
     Tabs3D &tb = p->tb;
     tb.tw_t = (const float2 *)(ws + L.tw_t);
+    tb.twA = (const float2 *)(ws + L.twA);
+    tb.twB = (const float2 *)(ws + L.twB);
     tb.tw_x = (const float2 *)(ws + L.tw_x);
     tb.tw_y = (const float2 *)(ws + L.tw_y);
     tb.d_w = (const float *)(ws + L.d_w);
