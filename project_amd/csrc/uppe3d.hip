@@ -67,7 +67,8 @@ struct Ctl3D {
 
 struct Tabs3D {
     const float2 *tw_t, *tw_x, *tw_y;
-    const float2 *twA, *twB;   // lane twiddles of the NT = 1024 line transform (tw1024_*)
+    const float2 *twA, *twB;   // lane twiddles of the NT = 1024 line transform (W1024)
+    const float2 *tw128;       // lane twiddles of the 128-point xy transforms (x128)
     const float *d_w, *k_w, *k2_w, *kref_w, *valid_w, *filt_w, *shock_w, *mask_w;
     const float *kx2, *ky2, *damp;
     const double *lut;
@@ -857,10 +858,8 @@ __device__ __forceinline__ int lslot(int e) {
 // written as D = i (d_w - kperp^2 / (kz + k)) for kz real (d_w = k - k_ref) and
 // D = -sqrt(kperp^2 - k^2) - i k_ref when evanescent; D = 0 where omega0 + omega <= 0
 // (solver_3d.py:56-87).
-template <int NY>
-__device__ __forceinline__ float2 eprop(int e, int w, float s, const Tabs3D &tb) {
+__device__ __forceinline__ float2 eprop_k(float kp2, int w, float s, const Tabs3D &tb) {
     if (tb.valid_w[w] == 0.0f) return make_float2(1.0f, 0.0f);
-    const float kp2 = tb.kx2[e / NY] + tb.ky2[e % NY];
     const float k2 = tb.k2_w[w];
     float dre = 0.0f, dim;
     if (kp2 <= k2) {
@@ -874,6 +873,11 @@ __device__ __forceinline__ float2 eprop(int e, int w, float s, const Tabs3D &tb)
     sincosf(s * dim, &sn, &cs);
     const float mag = dre == 0.0f ? 1.0f : expf(s * dre);
     return make_float2(mag * cs, mag * sn);
+}
+
+template <int NY>
+__device__ __forceinline__ float2 eprop(int e, int w, float s, const Tabs3D &tb) {
+    return eprop_k(tb.kx2[e / NY] + tb.ky2[e % NY], w, s, tb);
 }
 
 template <int T>
@@ -992,6 +996,190 @@ __global__ __launch_bounds__(xy_threads(NX, NY)) void xypass_kernel(Fields3D f, 
 }
 
 // ---------------------------------------------------------------------------------------
+// 128 x 128 planes.  Each length-128 transform (a row along y, or a column along x) is held
+// by 8 adjacent lanes, 16 values per lane, as two register DFT passes (16 x 8):
+//   n = j + 8 r (j = lane & 7, r < 16),  k = k1 + 16 k2 (k1 < 16, k2 < 8)
+//   forward: DFT16 over r -> x W128^(j k1) -> [wave-private LDS transpose] -> DFT8 over j.
+// After the forward pass lane (s = lane >> 3, jj = lane & 7) holds v[8 h + k2] = A[k1 + 16 k2],
+// k1 = 2 jj + h.  The inverse is the transposed sequence with conjugate twiddles.  Rows and
+// columns meet through one transpose of the whole plane in LDS.  Wave w takes rows
+// 8 w .. 8 w + 7 and columns 8 w .. 8 w + 7.  The k-space layout of a thread's values
+// (kx = 2 (lane & 7) + h + 16 k2, ky = 8 w + (lane >> 3)) is also the layout of the k-space
+// fields V and Q in HBM: value n of thread tid at n * 1024 + tid.
+// Lane twiddles: tw128[k * 64 + lane] = W128^((lane & 7) k).
+namespace x128 {
+constexpr int T = 1024;
+constexpr int PK = 65;                 // wave transpose: pitch per k1 (float2)
+constexpr int WREG = 16 * PK;          // float2 per wavefront
+constexpr int PP = 129;                // plane transpose: pitch per ky (float2)
+constexpr int LDS = 16 * WREG > 128 * PP ? 16 * WREG : 128 * PP;
+}  // namespace x128
+
+__device__ __forceinline__ void fwd128(float2 (&v)[16], float2 *wb, int lane,
+                                       const float2 *__restrict__ tw0) {
+    Dft<16, 1>::run(v);
+    const float2 *tw = opaque(tw0);
+#pragma unroll
+    for (int k = 1; k < 16; ++k) v[k] = cmul(v[k], tw[k * 64 + lane]);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) wb[k * x128::PK + lane] = v[k];
+    wave_lds_sync();
+    const int sq = lane >> 3, jj = lane & 7;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        float2 u[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) u[j] = wb[(2 * jj + h) * x128::PK + 8 * sq + j];
+        Dft<8, 1>::run(u);
+#pragma unroll
+        for (int k2 = 0; k2 < 8; ++k2) v[8 * h + k2] = u[k2];
+    }
+    wave_lds_sync();
+}
+
+__device__ __forceinline__ void inv128(float2 (&v)[16], float2 *wb, int lane,
+                                       const float2 *__restrict__ tw0) {
+    const int sq = lane >> 3, jj = lane & 7;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        float2 u[8];
+#pragma unroll
+        for (int k2 = 0; k2 < 8; ++k2) u[k2] = v[8 * h + k2];
+        Dft<8, -1>::run(u);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) wb[(2 * jj + h) * x128::PK + 8 * sq + j] = u[j];
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = wb[k * x128::PK + lane];
+    wave_lds_sync();
+    const float2 *tw = opaque(tw0);
+#pragma unroll
+    for (int k = 1; k < 16; ++k) v[k] = cmulc(v[k], tw[k * 64 + lane]);
+    Dft<16, -1>::run(v);
+}
+
+// row-transform output (lane: row 8 w + (lane >> 3), ky = 2 (lane & 7) + h + 16 k2) ->
+// column-transform input (lane: column ky = 8 w + (lane >> 3), x = (lane & 7) + 8 s)
+__device__ __forceinline__ void rows_to_cols(float2 (&v)[16], float2 *P, int wv, int lane) {
+    const int sq = lane >> 3, jj = lane & 7;
+    __syncthreads();   // wave transpose regions overlap the plane
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int k2 = 0; k2 < 8; ++k2)
+            P[(2 * jj + h + 16 * k2) * x128::PP + 8 * wv + sq] = v[8 * h + k2];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = P[(8 * wv + sq) * x128::PP + jj + 8 * r];
+    __syncthreads();
+}
+
+__device__ __forceinline__ void cols_to_rows(float2 (&v)[16], float2 *P, int wv, int lane) {
+    const int sq = lane >> 3, jj = lane & 7;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; ++r) P[(8 * wv + sq) * x128::PP + jj + 8 * r] = v[r];
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int k2 = 0; k2 < 8; ++k2)
+            v[8 * h + k2] = P[(2 * jj + h + 16 * k2) * x128::PP + 8 * wv + sq];
+    __syncthreads();
+}
+
+// plane (row-major, HBM) -> FFT2 in the k-space layout
+__device__ __forceinline__ void fft2_128(float2 (&v)[16], const float2 *__restrict__ src,
+                                         float2 *lds, int wv, int lane, const float2 *tw) {
+    const int row = 8 * wv + (lane >> 3), j = lane & 7;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = src[row * 128 + j + 8 * r];
+    fwd128(v, lds + wv * x128::WREG, lane, tw);
+    rows_to_cols(v, lds, wv, lane);
+    fwd128(v, lds + wv * x128::WREG, lane, tw);
+}
+
+// k-space layout -> iFFT2 / (128 * 128) -> plane (row-major, HBM)
+__device__ __forceinline__ void ifft2_128(float2 (&v)[16], float2 *__restrict__ dst,
+                                          float2 *lds, int wv, int lane, const float2 *tw) {
+    inv128(v, lds + wv * x128::WREG, lane, tw);
+    cols_to_rows(v, lds, wv, lane);
+    inv128(v, lds + wv * x128::WREG, lane, tw);
+    const int row = 8 * wv + (lane >> 3), j = lane & 7;
+    constexpr float s = 1.0f / 16384.0f;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) dst[row * 128 + j + 8 * r] = cscale(v[r], s);
+}
+
+// The xy passes of xypass_kernel (MODE 2..6, same arithmetic) for 128 x 128 planes.
+template <int MODE>
+__global__ __launch_bounds__(x128::T) void xy128_kernel(Fields3D f, Tabs3D tb, Ctl3D *ctl) {
+    __shared__ float2 lds[x128::LDS];
+    __shared__ unsigned int red[x128::T / 64];
+    if (ctl->done) return;
+    const int w = blockIdx.x;
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const size_t pb = (size_t)w * 16384;
+    const float h = (float)ctl->h;
+    float2 *const Xp = f.X + pb, *const Vp = f.V + pb, *const Qp = f.Q + pb;
+    const float2 *const Yp = (MODE == 5 ? f.Y4 : f.Yw) + pb;
+    const int ky = 8 * wv + (lane >> 3), kxb = 2 * (lane & 7);
+    float2 b[16];
+    fft2_128(b, Yp, lds, wv, lane, tb.tw128);
+    if (MODE == 6) {
+        unsigned int m = 0;
+#pragma unroll
+        for (int n = 0; n < 16; ++n) {
+            const unsigned int u = __float_as_uint(cabs2(b[n]));
+            m = u > m ? u : m;
+        }
+        block_max_to<x128::T>(m, &ctl->maxD_bits, red, tid);
+        return;
+    }
+    float2 a[16];
+    if (MODE == 2) {
+        const float2 *B = (ctl->cur ? f.B1 : f.B0) + pb;
+        fft2_128(a, B, lds, wv, lane, tb.tw128);
+        const float kyy = tb.ky2[ky];
+#pragma unroll
+        for (int n = 0; n < 16; ++n) {
+            const int kx = kxb + (n >> 3) + 16 * (n & 7);
+            const float2 E = eprop_k(tb.kx2[kx] + kyy, w, 0.5f * h, tb);
+            Vp[n * x128::T + tid] = cmul(E, a[n]);
+            Qp[n * x128::T + tid] = cmul(E, caxpy(a[n], h / 6.0f, b[n]));
+            a[n] = cmul(E, caxpy(a[n], 0.5f * h, b[n]));
+        }
+    } else if (MODE == 3 || MODE == 4) {
+        const float kyy = tb.ky2[ky];
+#pragma unroll
+        for (int n = 0; n < 16; ++n) {
+            const float2 vv = Vp[n * x128::T + tid];
+            const float2 qv = Qp[n * x128::T + tid];
+            if (MODE == 3) {
+                a[n] = caxpy(vv, 0.5f * h, b[n]);
+                Qp[n * x128::T + tid] = caxpy(qv, h / 3.0f, b[n]);
+            } else {
+                const int kx = kxb + (n >> 3) + 16 * (n & 7);
+                const float2 E = eprop_k(tb.kx2[kx] + kyy, w, 0.5f * h, tb);
+                a[n] = cmul(E, caxpy(vv, h, b[n]));
+                Qp[n * x128::T + tid] = cmul(E, caxpy(qv, h / 3.0f, b[n]));
+            }
+        }
+    } else {  // MODE 5
+        unsigned int m = 0;
+#pragma unroll
+        for (int n = 0; n < 16; ++n) {
+            a[n] = caxpy(Qp[n * x128::T + tid], h / 6.0f, b[n]);
+            const unsigned int u = __float_as_uint(cabs2(a[n]));
+            m = u > m ? u : m;
+        }
+        block_max_to<x128::T>(m, &ctl->maxA_bits, red, tid);
+    }
+    ifft2_128(a, Xp, lds, wv, lane, tb.tw128);
+}
+
+// ---------------------------------------------------------------------------------------
 // Step controller (solver_3d.py:182-208), one thread, FP64.  Also applies the top-of-loop
 // clip h = min(h, L - z) of the next iteration and the h < 1e-15 stop.
 __global__ void ctl_kernel(Ctl3D *c) {
@@ -1100,7 +1288,7 @@ constexpr size_t kAlign = 256;
 size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 struct Layout3D {
-    size_t tw_t, tw_x, tw_y, twA, twB, d_w, k_w, k2_w, kref_w, valid_w, filt_w, shock_w, mask_w, kx2, ky2,
+    size_t tw_t, tw_x, tw_y, twA, twB, tw128, d_w, k_w, k2_w, kref_w, valid_w, filt_w, shock_w, mask_w, kx2, ky2,
         damp, lut, ctl, list, fields[7], total;
 };
 
@@ -1115,6 +1303,7 @@ Layout3D layout3d(const uppe3d_desc &d) {
     L.tw_t = take(sizeof(float2) * d.nt);
     L.twA = take(sizeof(float2) * 16 * 64);
     L.twB = take(sizeof(float2) * 16 * 64);
+    L.tw128 = take(sizeof(float2) * 16 * 64);
     L.tw_x = take(sizeof(float2) * d.nx);
     L.tw_y = take(sizeof(float2) * d.ny);
     L.d_w = take(4 * d.nt);
@@ -1213,7 +1402,18 @@ int launch_xy(uppe3d_plan *p, int mode, hipStream_t s) {
         case 16: return launch_xy_n<16>(p, mode, s);
         case 32: return launch_xy_n<32>(p, mode, s);
         case 64: return launch_xy_n<64>(p, mode, s);
-        case 128: return launch_xy_n<128>(p, mode, s);
+        case 128: {
+            const dim3 grid(p->d.nt), block(x128::T);
+            switch (mode) {
+                case 2: xy128_kernel<2><<<grid, block, 0, s>>>(p->f, p->tb, p->ctl); break;
+                case 3: xy128_kernel<3><<<grid, block, 0, s>>>(p->f, p->tb, p->ctl); break;
+                case 4: xy128_kernel<4><<<grid, block, 0, s>>>(p->f, p->tb, p->ctl); break;
+                case 5: xy128_kernel<5><<<grid, block, 0, s>>>(p->f, p->tb, p->ctl); break;
+                default: xy128_kernel<6><<<grid, block, 0, s>>>(p->f, p->tb, p->ctl); break;
+            }
+            UPPE_HIP(hipGetLastError());
+            return UPPE_OK;
+        }
     }
     return UPPE_EINVAL;
 }
@@ -1310,14 +1510,19 @@ int uppe3d_plan_create(const uppe3d_desc *d, void *workspace, size_t workspace_b
     UPPE_HIP(hipMemcpy(ws + L.tw_t, twt.data(), sizeof(float2) * nt, hipMemcpyHostToDevice));
     {
         // lane twiddles of the NT = 1024 line transform (W1024): rows of 64 lanes
-        std::vector<float2> ta(16 * 64), tbl(16 * 64);
+        // and of the 128-point xy transforms (x128)
+        std::vector<float2> ta(16 * 64), tbl(16 * 64), t128(16 * 64);
         for (int k = 0; k < 16; ++k)
             for (int l = 0; l < 64; ++l) {
                 const double a = -2.0 * M_PI * (double)(l * k) / 1024.0;
                 const double b = -2.0 * M_PI * (double)((l & 3) * k) / 64.0;
+                const double c = -2.0 * M_PI * (double)((l & 7) * k) / 128.0;
                 ta[k * 64 + l] = make_float2((float)cos(a), (float)sin(a));
                 tbl[k * 64 + l] = make_float2((float)cos(b), (float)sin(b));
+                t128[k * 64 + l] = make_float2((float)cos(c), (float)sin(c));
             }
+        UPPE_HIP(hipMemcpy(ws + L.tw128, t128.data(), sizeof(float2) * t128.size(),
+                           hipMemcpyHostToDevice));
         UPPE_HIP(hipMemcpy(ws + L.twA, ta.data(), sizeof(float2) * ta.size(),
                            hipMemcpyHostToDevice));
         UPPE_HIP(hipMemcpy(ws + L.twB, tbl.data(), sizeof(float2) * tbl.size(),
@@ -1360,6 +1565,7 @@ int uppe3d_plan_create(const uppe3d_desc *d, void *workspace, size_t workspace_b
     tb.tw_t = (const float2 *)(ws + L.tw_t);
     tb.twA = (const float2 *)(ws + L.twA);
     tb.twB = (const float2 *)(ws + L.twB);
+    tb.tw128 = (const float2 *)(ws + L.tw128);
     tb.tw_x = (const float2 *)(ws + L.tw_x);
     tb.tw_y = (const float2 *)(ws + L.tw_y);
     tb.d_w = (const float *)(ws + L.d_w);

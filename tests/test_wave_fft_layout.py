@@ -128,3 +128,88 @@ def test_w1024_index_algebra():
     V = w1024_forward(y[tix], twA, twB)
     np.testing.assert_allclose(V, np.fft.fft(y)[np.arange(64)[:, None] + 64 * np.arange(16)[None, :]],
                                atol=1e-10)
+
+
+# -- 128 x 128 plane transform: 16 x 8 register DFTs per 8 lanes, wave and plane transposes --
+PK, WREG, PP = 65, 16 * 65, 129
+
+
+def tw128():
+    k = np.arange(16)[:, None]
+    lane = np.arange(64)[None, :]
+    return np.exp(-2j * np.pi * ((lane & 7) * k) / 128)    # [k, lane]
+
+
+def fwd128(v, tw):
+    """v[lane, r] = a_(lane>>3)[(lane & 7) + 8 r] -> v[lane, 8 h + k2] = A_(lane>>3)[k1 + 16 k2],
+    k1 = 2 (lane & 7) + h."""
+    lane = np.arange(64)
+    v = np.fft.fft(v, axis=1) * tw.T
+    wb = np.full(WREG, np.nan + 0j)
+    addr = np.arange(16)[None, :] * PK + lane[:, None]
+    assert len(np.unique(addr)) == addr.size and addr.max() < WREG
+    wb[addr] = v
+    sq, jj = lane >> 3, lane & 7
+    out = np.zeros((64, 16), complex)
+    for h in range(2):
+        u = wb[(2 * jj + h)[:, None] * PK + 8 * sq[:, None] + np.arange(8)[None, :]]
+        out[:, 8 * h:8 * h + 8] = np.fft.fft(u, axis=1)
+    assert np.isfinite(out).all()
+    return out
+
+
+def inv128(v, tw):
+    lane = np.arange(64)
+    sq, jj = lane >> 3, lane & 7
+    wb = np.full(WREG, np.nan + 0j)
+    for h in range(2):
+        u = np.fft.ifft(v[:, 8 * h:8 * h + 8], axis=1) * 8
+        wb[(2 * jj + h)[:, None] * PK + 8 * sq[:, None] + np.arange(8)[None, :]] = u
+    v = wb[np.arange(16)[None, :] * PK + lane[:, None]]
+    v = np.fft.ifft(v * np.conj(tw.T), axis=1) * 16
+    assert np.isfinite(v).all()
+    return v
+
+
+def test_x128_plane_algebra():
+    tw = tw128()
+    rng = np.random.default_rng(11)
+    A = rng.standard_normal((128, 128)) + 1j * rng.standard_normal((128, 128))
+    lane = np.arange(64)
+    sq, jj = lane >> 3, lane & 7
+    P = np.full(128 * PP, np.nan + 0j)
+    regs = []
+    for wv in range(16):
+        row = 8 * wv + sq
+        v = A[row[:, None], jj[:, None] + 8 * np.arange(16)[None, :]]
+        v = fwd128(v, tw)
+        for h in range(2):
+            for k2 in range(8):
+                addr = (2 * jj + h + 16 * k2) * PP + 8 * wv + sq
+                P[addr] = v[:, 8 * h + k2]
+    assert np.isfinite(P[[ky * PP + x for ky in range(128) for x in range(128)]]).all()
+    K = np.zeros((128, 128), complex)          # K[kx, ky]
+    for wv in range(16):
+        v = P[(8 * wv + sq)[:, None] * PP + jj[:, None] + 8 * np.arange(16)[None, :]]
+        v = fwd128(v, tw)
+        regs.append(v)
+        ky = 8 * wv + sq
+        for n in range(16):
+            kx = 2 * jj + (n >> 3) + 16 * (n & 7)
+            K[kx, ky] = v[:, n]
+    np.testing.assert_allclose(K, np.fft.fft2(A), atol=1e-9)
+    # inverse back to rows
+    P = np.full(128 * PP, np.nan + 0j)
+    for wv in range(16):
+        v = inv128(regs[wv], tw)
+        P[(8 * wv + sq)[:, None] * PP + jj[:, None] + 8 * np.arange(16)[None, :]] = v
+    out = np.zeros((128, 128), complex)
+    for wv in range(16):
+        v = np.zeros((64, 16), complex)
+        for h in range(2):
+            for k2 in range(8):
+                v[:, 8 * h + k2] = P[(2 * jj + h + 16 * k2) * PP + 8 * wv + sq]
+        v = inv128(v, tw) / 16384
+        out[(8 * wv + sq)[:, None], jj[:, None] + 8 * np.arange(16)[None, :]] = v
+    np.testing.assert_allclose(out, A, atol=1e-12)
+    assert max(16 * WREG, 128 * PP) * 8 <= 160 * 1024
