@@ -72,6 +72,8 @@ struct Tabs3D {
     const float *d_w, *k_w, *k2_w, *kref_w, *valid_w, *filt_w, *shock_w, *mask_w;
     const float *kx2, *ky2, *damp;
     const double *lut;
+    const double *d64, *kref64;   // k - k_ref and k_ref (FP64) for the per-plane phases
+    float2 *ph_d, *ph_r;          // exp(i s d_w), exp(-i s k_ref) for s = h / 2 (phase_kernel)
 };
 
 struct Phys3D {
@@ -854,30 +856,65 @@ __device__ __forceinline__ int lslot(int e) {
     return (e / NY) * (NY + 1) + (e % NY);
 }
 
-// exp(s * D(kx, ky, omega_w)) with D = i (kz - k_ref), kz the principal sqrt of k^2 - kperp^2,
-// written as D = i (d_w - kperp^2 / (kz + k)) for kz real (d_w = k - k_ref) and
-// D = -sqrt(kperp^2 - k^2) - i k_ref when evanescent; D = 0 where omega0 + omega <= 0
-// (solver_3d.py:56-87).
-__device__ __forceinline__ float2 eprop_k(float kp2, int w, float s, const Tabs3D &tb) {
+// exp(i phi) of an FP64 phase, rounded to float (per-plane factors)
+__device__ __forceinline__ float2 cis_d(double phi) {
+    double sn, cs;
+    sincos(phi, &sn, &cs);
+    return make_float2((float)cs, (float)sn);
+}
+
+// exp(i x) for float x, |x| up to ~1e5: x = r + k pi/2 with an FMA Cody-Waite reduction
+// (pi/2 = hi + lo, hi = float(pi/2)), then Taylor polynomials on |r| <= pi/4 (truncation
+// below 2e-9) and the quadrant rotation i^k.
+__device__ __forceinline__ float2 cis_f(float x) {
+    const float k = rintf(x * 0.636619772367581343f);
+    float r = fmaf(-k, 1.57079637050628662f, x);
+    r = fmaf(-k, -4.37113900018624283e-8f, r);
+    const float r2 = r * r;
+    float sp = fmaf(r2, 1.0f / 362880.0f, -1.0f / 5040.0f);
+    sp = fmaf(r2, sp, 1.0f / 120.0f);
+    sp = fmaf(r2, sp, -1.0f / 6.0f);
+    const float sn = fmaf(r * r2, sp, r);
+    float cp = fmaf(r2, -1.0f / 3628800.0f, 1.0f / 40320.0f);
+    cp = fmaf(r2, cp, -1.0f / 720.0f);
+    cp = fmaf(r2, cp, 1.0f / 24.0f);
+    cp = fmaf(r2, cp, -0.5f);
+    const float cs = fmaf(r2, cp, 1.0f);
+    const int q = (int)k & 3;
+    const float c = (q & 1) ? sn : cs, sv = (q & 1) ? cs : sn;
+    return make_float2((q == 1 || q == 2) ? -c : c, (q >= 2) ? -sv : sv);
+}
+
+// exp(s * D(kx, ky, omega_w)) with D = i (kz - k_ref), kz the principal sqrt of k^2 - kperp^2
+// (solver_3d.py:56-87): for kz real D = i (d_w - kperp^2 / (kz + k)) (d_w = k - k_ref), for
+// evanescent kz D = -sqrt(kperp^2 - k^2) - i k_ref, and D = 0 where omega0 + omega <= 0.
+// The per-plane factors ed = exp(i s d_w), eref = exp(-i s k_ref) come from the caller (FP64
+// phases, phase_kernel); the per-element phase s kperp^2 / (kz + k) is small.
+__device__ __forceinline__ float2 eprop_k(float kp2, int w, float s, const Tabs3D &tb,
+                                          float2 ed, float2 eref) {
     if (tb.valid_w[w] == 0.0f) return make_float2(1.0f, 0.0f);
     const float k2 = tb.k2_w[w];
-    float dre = 0.0f, dim;
     if (kp2 <= k2) {
         const float kz = sqrtf(k2 - kp2);
-        dim = tb.d_w[w] - kp2 / (kz + tb.k_w[w]);
-    } else {
-        dre = -sqrtf(kp2 - k2);
-        dim = -tb.kref_w[w];
+        return cmul(ed, cis_f(-s * kp2 * __builtin_amdgcn_rcpf(kz + tb.k_w[w])));
     }
-    float sn, cs;
-    sincosf(s * dim, &sn, &cs);
-    const float mag = dre == 0.0f ? 1.0f : expf(s * dre);
-    return make_float2(mag * cs, mag * sn);
+    return cscale(eref, expf(-s * sqrtf(kp2 - k2)));
+}
+
+// Per-plane factors exp(i s d_w), exp(-i s k_ref) for the current step h (s = h / 2), phases
+// in FP64.  Launched whenever h is set (begin, controller, replay).
+__global__ void phase_kernel(const Ctl3D *ctl, Tabs3D tb, int nt) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nt) return;
+    const double s = 0.5 * ctl->h;
+    tb.ph_d[w] = cis_d(s * tb.d64[w]);
+    tb.ph_r[w] = cis_d(-s * tb.kref64[w]);
 }
 
 template <int NY>
-__device__ __forceinline__ float2 eprop(int e, int w, float s, const Tabs3D &tb) {
-    return eprop_k(tb.kx2[e / NY] + tb.ky2[e % NY], w, s, tb);
+__device__ __forceinline__ float2 eprop(int e, int w, float s, const Tabs3D &tb, float2 ed,
+                                        float2 eref) {
+    return eprop_k(tb.kx2[e / NY] + tb.ky2[e % NY], w, s, tb, ed, eref);
 }
 
 template <int T>
@@ -916,6 +953,7 @@ __global__ __launch_bounds__(xy_threads(NX, NY)) void xypass_kernel(Fields3D f, 
     const size_t pb = (size_t)w * C::P;
     const float h = (float)ctl->h;
     const float inv = 1.0f / (float)(NX * NY);
+    const float2 ed = tb.ph_d[w], eref = tb.ph_r[w];
     // per-plane base pointers (uniform) and 32-bit in-plane offsets
     float2 *const Xp = f.X + pb, *const Vp = f.V + pb, *const Qp = f.Q + pb;
     const float2 *const Yp = (MODE == 5 ? f.Y4 : f.Yw) + pb;
@@ -953,7 +991,7 @@ __global__ __launch_bounds__(xy_threads(NX, NY)) void xypass_kernel(Fields3D f, 
         for (int i = 0; i < C::EPT; ++i) {
             const int e = tid + C::T * i;
             const float2 a = lds[lslot<NY>(e)];
-            const float2 E = eprop<NY>(e, w, 0.5f * h, tb);
+            const float2 E = eprop<NY>(e, w, 0.5f * h, tb, ed, eref);
             Vp[e] = cmul(E, a);
             Qp[e] = cmul(E, caxpy(a, h / 6.0f, b[i]));
             lds[lslot<NY>(e)] = cmul(E, caxpy(a, 0.5f * h, b[i]));
@@ -969,7 +1007,7 @@ __global__ __launch_bounds__(xy_threads(NX, NY)) void xypass_kernel(Fields3D f, 
                 lds[lslot<NY>(e)] = caxpy(v, 0.5f * h, bh);
                 Qp[e] = caxpy(qv, h / 3.0f, bh);
             } else {
-                const float2 E = eprop<NY>(e, w, 0.5f * h, tb);
+                const float2 E = eprop<NY>(e, w, 0.5f * h, tb, ed, eref);
                 lds[lslot<NY>(e)] = cmul(E, caxpy(v, h, bh));
                 Qp[e] = cmul(E, caxpy(qv, h / 3.0f, bh));
             }
@@ -1015,12 +1053,23 @@ constexpr int PP = 129;                // plane transpose: pitch per ky (float2)
 constexpr int LDS = 16 * WREG > 128 * PP ? 16 * WREG : 128 * PP;
 }  // namespace x128
 
+// Lane twiddles applied in two groups of loads (bounded registers in flight).
+template <int DIR>
+__device__ __forceinline__ void twiddle128(float2 (&v)[16], int lane,
+                                           const float2 *__restrict__ tw0) {
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+        const float2 *tw = opaque(tw0);
+#pragma unroll
+        for (int k = (g == 0 ? 1 : 8); k < (g == 0 ? 8 : 16); ++k)
+            v[k] = DIR > 0 ? cmul(v[k], tw[k * 64 + lane]) : cmulc(v[k], tw[k * 64 + lane]);
+    }
+}
+
 __device__ __forceinline__ void fwd128(float2 (&v)[16], float2 *wb, int lane,
                                        const float2 *__restrict__ tw0) {
     Dft<16, 1>::run(v);
-    const float2 *tw = opaque(tw0);
-#pragma unroll
-    for (int k = 1; k < 16; ++k) v[k] = cmul(v[k], tw[k * 64 + lane]);
+    twiddle128<1>(v, lane, tw0);
 #pragma unroll
     for (int k = 0; k < 16; ++k) wb[k * x128::PK + lane] = v[k];
     wave_lds_sync();
@@ -1053,10 +1102,15 @@ __device__ __forceinline__ void inv128(float2 (&v)[16], float2 *wb, int lane,
 #pragma unroll
     for (int k = 0; k < 16; ++k) v[k] = wb[k * x128::PK + lane];
     wave_lds_sync();
-    const float2 *tw = opaque(tw0);
-#pragma unroll
-    for (int k = 1; k < 16; ++k) v[k] = cmulc(v[k], tw[k * 64 + lane]);
+    twiddle128<-1>(v, lane, tw0);
     Dft<16, -1>::run(v);
+}
+
+// An int the optimiser cannot see through: addresses derived from it are recomputed where
+// used instead of being held in registers across the kernel.
+__device__ __forceinline__ int opaque_i(int x) {
+    asm volatile("" : "+v"(x));
+    return x;
 }
 
 // row-transform output (lane: row 8 w + (lane >> 3), ky = 2 (lane & 7) + h + 16 k2) ->
@@ -1064,28 +1118,30 @@ __device__ __forceinline__ void inv128(float2 (&v)[16], float2 *wb, int lane,
 __device__ __forceinline__ void rows_to_cols(float2 (&v)[16], float2 *P, int wv, int lane) {
     const int sq = lane >> 3, jj = lane & 7;
     __syncthreads();   // wave transpose regions overlap the plane
+    const int wr = opaque_i(2 * jj * x128::PP + 8 * wv + sq);
 #pragma unroll
     for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int k2 = 0; k2 < 8; ++k2)
-            P[(2 * jj + h + 16 * k2) * x128::PP + 8 * wv + sq] = v[8 * h + k2];
+        for (int k2 = 0; k2 < 8; ++k2) P[wr + (h + 16 * k2) * x128::PP] = v[8 * h + k2];
     __syncthreads();
+    const int rd = opaque_i((8 * wv + sq) * x128::PP + jj);
 #pragma unroll
-    for (int r = 0; r < 16; ++r) v[r] = P[(8 * wv + sq) * x128::PP + jj + 8 * r];
+    for (int r = 0; r < 16; ++r) v[r] = P[rd + 8 * r];
     __syncthreads();
 }
 
 __device__ __forceinline__ void cols_to_rows(float2 (&v)[16], float2 *P, int wv, int lane) {
     const int sq = lane >> 3, jj = lane & 7;
     __syncthreads();
+    const int wr = opaque_i((8 * wv + sq) * x128::PP + jj);
 #pragma unroll
-    for (int r = 0; r < 16; ++r) P[(8 * wv + sq) * x128::PP + jj + 8 * r] = v[r];
+    for (int r = 0; r < 16; ++r) P[wr + 8 * r] = v[r];
     __syncthreads();
+    const int rd = opaque_i(2 * jj * x128::PP + 8 * wv + sq);
 #pragma unroll
     for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int k2 = 0; k2 < 8; ++k2)
-            v[8 * h + k2] = P[(2 * jj + h + 16 * k2) * x128::PP + 8 * wv + sq];
+        for (int k2 = 0; k2 < 8; ++k2) v[8 * h + k2] = P[rd + (h + 16 * k2) * x128::PP];
     __syncthreads();
 }
 
@@ -1122,6 +1178,7 @@ __global__ __launch_bounds__(x128::T) void xy128_kernel(Fields3D f, Tabs3D tb, C
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const size_t pb = (size_t)w * 16384;
     const float h = (float)ctl->h;
+    const float2 ed = tb.ph_d[w], eref = tb.ph_r[w];
     float2 *const Xp = f.X + pb, *const Vp = f.V + pb, *const Qp = f.Q + pb;
     const float2 *const Yp = (MODE == 5 ? f.Y4 : f.Yw) + pb;
     const int ky = 8 * wv + (lane >> 3), kxb = 2 * (lane & 7);
@@ -1145,10 +1202,11 @@ __global__ __launch_bounds__(x128::T) void xy128_kernel(Fields3D f, Tabs3D tb, C
 #pragma unroll
         for (int n = 0; n < 16; ++n) {
             const int kx = kxb + (n >> 3) + 16 * (n & 7);
-            const float2 E = eprop_k(tb.kx2[kx] + kyy, w, 0.5f * h, tb);
+            const float2 E = eprop_k(tb.kx2[kx] + kyy, w, 0.5f * h, tb, ed, eref);
             Vp[n * x128::T + tid] = cmul(E, a[n]);
             Qp[n * x128::T + tid] = cmul(E, caxpy(a[n], h / 6.0f, b[n]));
             a[n] = cmul(E, caxpy(a[n], 0.5f * h, b[n]));
+            __builtin_amdgcn_sched_barrier(0);   // one element at a time: bounded registers
         }
     } else if (MODE == 3 || MODE == 4) {
         const float kyy = tb.ky2[ky];
@@ -1161,9 +1219,10 @@ __global__ __launch_bounds__(x128::T) void xy128_kernel(Fields3D f, Tabs3D tb, C
                 Qp[n * x128::T + tid] = caxpy(qv, h / 3.0f, b[n]);
             } else {
                 const int kx = kxb + (n >> 3) + 16 * (n & 7);
-                const float2 E = eprop_k(tb.kx2[kx] + kyy, w, 0.5f * h, tb);
+                const float2 E = eprop_k(tb.kx2[kx] + kyy, w, 0.5f * h, tb, ed, eref);
                 a[n] = cmul(E, caxpy(vv, h, b[n]));
                 Qp[n * x128::T + tid] = cmul(E, caxpy(qv, h / 3.0f, b[n]));
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
     } else {  // MODE 5
@@ -1288,7 +1347,7 @@ constexpr size_t kAlign = 256;
 size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 struct Layout3D {
-    size_t tw_t, tw_x, tw_y, twA, twB, tw128, d_w, k_w, k2_w, kref_w, valid_w, filt_w, shock_w, mask_w, kx2, ky2,
+    size_t tw_t, tw_x, tw_y, twA, twB, tw128, d64, kref64, ph_d, ph_r, d_w, k_w, k2_w, kref_w, valid_w, filt_w, shock_w, mask_w, kx2, ky2,
         damp, lut, ctl, list, fields[7], total;
 };
 
@@ -1306,6 +1365,10 @@ Layout3D layout3d(const uppe3d_desc &d) {
     L.tw128 = take(sizeof(float2) * 16 * 64);
     L.tw_x = take(sizeof(float2) * d.nx);
     L.tw_y = take(sizeof(float2) * d.ny);
+    L.d64 = take(8 * d.nt);
+    L.kref64 = take(8 * d.nt);
+    L.ph_d = take(8 * d.nt);
+    L.ph_r = take(8 * d.nt);
     L.d_w = take(4 * d.nt);
     L.k_w = take(4 * d.nt);
     L.k2_w = take(4 * d.nt);
@@ -1455,6 +1518,7 @@ int enqueue_attempt(uppe3d_plan *p, hipStream_t s, hipEvent_t *ev = nullptr) {
     RET(launch_xy(p, 6, s));                          // P6
     RET(mark());
     ctl_kernel<<<1, 1, 0, s>>>(p->ctl);
+    phase_kernel<<<(p->d.nt + 255) / 256, 256, 0, s>>>(p->ctl, p->tb, p->d.nt);
     UPPE_HIP(hipGetLastError());
     RET(mark());
     return UPPE_OK;
@@ -1492,7 +1556,9 @@ int uppe3d_plan_create(const uppe3d_desc *d, void *workspace, size_t workspace_b
     twiddles(twx, nx);
     twiddles(twy, ny);
     std::vector<float> dw(nt), kw(nt), k2w(nt), krw(nt), vw(nt), fw(nt), sw(nt), mw(nt);
+    std::vector<double> d64(nt);
     for (int i = 0; i < nt; ++i) {
+        d64[i] = k_w[i] - kref_w[i];
         dw[i] = (float)(k_w[i] - kref_w[i]);
         kw[i] = (float)k_w[i];
         k2w[i] = (float)(k_w[i] * k_w[i]);
@@ -1531,6 +1597,8 @@ int uppe3d_plan_create(const uppe3d_desc *d, void *workspace, size_t workspace_b
     UPPE_HIP(hipMemcpy(ws + L.tw_x, twx.data(), sizeof(float2) * nx, hipMemcpyHostToDevice));
     UPPE_HIP(hipMemcpy(ws + L.tw_y, twy.data(), sizeof(float2) * ny, hipMemcpyHostToDevice));
     UPPE_HIP(hipMemcpy(ws + L.d_w, dw.data(), 4 * nt, hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(ws + L.d64, d64.data(), 8 * nt, hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(ws + L.kref64, kref_w, 8 * nt, hipMemcpyHostToDevice));
     UPPE_HIP(hipMemcpy(ws + L.k_w, kw.data(), 4 * nt, hipMemcpyHostToDevice));
     UPPE_HIP(hipMemcpy(ws + L.k2_w, k2w.data(), 4 * nt, hipMemcpyHostToDevice));
     UPPE_HIP(hipMemcpy(ws + L.kref_w, krw.data(), 4 * nt, hipMemcpyHostToDevice));
@@ -1569,6 +1637,10 @@ int uppe3d_plan_create(const uppe3d_desc *d, void *workspace, size_t workspace_b
     tb.tw_x = (const float2 *)(ws + L.tw_x);
     tb.tw_y = (const float2 *)(ws + L.tw_y);
     tb.d_w = (const float *)(ws + L.d_w);
+    tb.d64 = (const double *)(ws + L.d64);
+    tb.kref64 = (const double *)(ws + L.kref64);
+    tb.ph_d = (float2 *)(ws + L.ph_d);
+    tb.ph_r = (float2 *)(ws + L.ph_r);
     tb.k_w = (const float *)(ws + L.k_w);
     tb.k2_w = (const float *)(ws + L.k2_w);
     tb.kref_w = (const float *)(ws + L.kref_w);
@@ -1698,6 +1770,7 @@ int uppe3d_begin(uppe3d_plan *p, double L, double tol, double max_step, double m
     UPPE_CHECK(tol > 0.0 && max_step > 0.0 && min_step >= 0.0,
                "uppe3d_begin: tol and max_step must be positive");
     begin_kernel<<<1, 1, 0, (hipStream_t)stream>>>(p->ctl, L, tol, max_step, min_step);
+    phase_kernel<<<(p->d.nt + 255) / 256, 256, 0, (hipStream_t)stream>>>(p->ctl, p->tb, p->d.nt);
     UPPE_HIP(hipGetLastError());
     return UPPE_OK;
 }
@@ -1743,6 +1816,7 @@ int uppe3d_replay_attempt(uppe3d_plan *p, double h, int accept, void *stream) {
     UPPE_CHECK(p != nullptr && h > 0.0, "uppe3d_replay_attempt: bad argument");
     hipStream_t s = (hipStream_t)stream;
     replay_set_kernel<<<1, 1, 0, s>>>(p->ctl, h, accept, h);
+    phase_kernel<<<(p->d.nt + 255) / 256, 256, 0, s>>>(p->ctl, p->tb, p->d.nt);
     UPPE_HIP(hipGetLastError());
     return enqueue_attempt(p, s);
 }

@@ -213,3 +213,41 @@ def test_x128_plane_algebra():
         out[(8 * wv + sq)[:, None], jj[:, None] + 8 * np.arange(16)[None, :]] = v
     np.testing.assert_allclose(out, A, atol=1e-12)
     assert max(16 * WREG, 128 * PP) * 8 <= 160 * 1024
+
+
+def cis_f32(x):
+    """Float32 restatement of the device exp(i x) (FMA Cody-Waite reduction by pi/2, Taylor
+    polynomials, quadrant rotation); FMAs are emulated in float64 then rounded."""
+    f = np.float32
+    x = np.asarray(x, np.float32)
+    k = np.rint(x * f(0.636619772367581343)).astype(np.float32)
+    r = (-k.astype(np.float64) * np.float64(f(1.57079637050628662)) + x).astype(np.float32)
+    r = (-k.astype(np.float64) * np.float64(f(-4.37113900018624283e-8)) + r).astype(np.float32)
+    r2 = r * r
+
+    def fma(a, b, c):
+        return (np.float64(a) * np.float64(b) + np.float64(c)).astype(np.float32)
+    sp = fma(r2, f(1 / 362880), f(-1 / 5040))
+    sp = fma(r2, sp, f(1 / 120))
+    sp = fma(r2, sp, f(-1 / 6))
+    sn = fma(r * r2, sp, r)
+    cp = fma(r2, f(-1 / 3628800), f(1 / 40320))
+    cp = fma(r2, cp, f(-1 / 720))
+    cp = fma(r2, cp, f(1 / 24))
+    cp = fma(r2, cp, f(-0.5))
+    cs = fma(r2, cp, f(1))
+    q = k.astype(np.int64) & 3
+    c = np.where(q & 1, sn, cs)
+    s = np.where(q & 1, cs, sn)
+    return np.where((q == 1) | (q == 2), -c, c) + 1j * np.where(q >= 2, -s, s)
+
+
+def test_cis_f32_accuracy():
+    rng = np.random.default_rng(5)
+    for scale in (1e-3, 1.0, 30.0, 3e3, 1e5):
+        x = (rng.uniform(-1, 1, 20000) * scale).astype(np.float32)
+        got = cis_f32(x)
+        ref = np.exp(1j * x.astype(np.float64))
+        err = np.abs(got - ref)
+        # a few float32 ulps of the result plus the reduction error k * 2e-15
+        assert err.max() < 4e-7 + 2e-15 * scale, (scale, err.max())
