@@ -217,6 +217,13 @@ __device__ __forceinline__ const T *opaque(const T *p) {
     return p;
 }
 
+// An int the optimiser cannot see through: addresses derived from it are recomputed where
+// used instead of being held in registers across the kernel.
+__device__ __forceinline__ int opaque_i(int x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
 // x[r] = X(omega = l + 64 r), l = bitrev6(lane)  ->  x[k] = A(t = k + R lane), scaled by 1/NT
 template <int R>
 __device__ __forceinline__ void wave_ifft_t(float2 (&x)[R], int lane, int rl,
@@ -493,14 +500,17 @@ __device__ __forceinline__ void line_forward(float2 (&v)[R], float2 *buf, int la
 template <int NT, int LINES>
 struct TStage {
     static constexpr int R = NT / 64;
-    static constexpr int CH = R < 4 ? R : 4;
     static constexpr int SP = 66;   // padded stage row (float2): q-stride of 33 dword pairs
-    static constexpr int SIZE = CH * LINES * SP;
     // lean NT = 1024 instantiation: the line transforms run through LDS (W1024), in a region
-    // shared with the HBM stage (separated by workgroup barriers)
-    static constexpr bool fast(bool plasma) { return NT == 1024 && !plasma; }
+    // shared with the HBM stage (separated by workgroup barriers); that region holds the
+    // whole group, so all rows are loaded (and stored) in one chunk.  Otherwise the stage
+    // holds CH = 4 chunks of 64 rows, double-buffered in registers.
+    static constexpr bool fast(bool plasma) { return NT == 1024 && LINES == 16 && !plasma; }
+    static constexpr int ch(bool plasma) { return fast(plasma) ? R : (R < 4 ? R : 4); }
     static constexpr int lds(bool plasma) {
-        return fast(plasma) && LINES * W1024::REGION > SIZE ? LINES * W1024::REGION : SIZE;
+        return fast(plasma) && LINES * W1024::REGION > ch(plasma) * LINES * SP
+                   ? LINES * W1024::REGION
+                   : ch(plasma) * LINES * SP;
     }
 };
 
@@ -510,7 +520,7 @@ __device__ __forceinline__ void tpass_group(int blk, float2 *stage, Fields3D f,
                                             float2 *out, const Tabs3D &tb, const Phys3D &ph,
                                             Ctl3D *ctl, int nx, int ny, int *list, int slot) {
     constexpr int R = TStage<NT, LINES>::R;
-    constexpr int CH = TStage<NT, LINES>::CH;
+    constexpr int CH = TStage<NT, LINES>::ch(PLASMA);
     constexpr int SP = TStage<NT, LINES>::SP;
     constexpr bool FAST = TStage<NT, LINES>::fast(PLASMA);
     const int cur = ctl->cur;
@@ -555,11 +565,13 @@ __device__ __forceinline__ void tpass_group(int blk, float2 *stage, Fields3D f,
                     bnew[lo + 64 * (c0 + cc) * plane] = cscale(g[cc], tb.filt_w[w] * absorb);
                 }
             }
+            const int sw = opaque_i(q * SP + row);
 #pragma unroll
-            for (int cc = 0; cc < CH; ++cc) stage[(cc * LINES + q) * SP + row] = g[cc];
+            for (int cc = 0; cc < CH; ++cc) stage[sw + cc * LINES * SP] = g[cc];
             __syncthreads();
+            const int sr = opaque_i(wv * SP + sl);
 #pragma unroll
-            for (int cc = 0; cc < CH; ++cc) v[c0 + cc] = stage[(cc * LINES + wv) * SP + sl];
+            for (int cc = 0; cc < CH; ++cc) v[c0 + cc] = stage[sr + cc * LINES * SP];
             __syncthreads();
 #pragma unroll
             for (int cc = 0; cc < CH; ++cc) g[cc] = gn[cc];
@@ -647,13 +659,15 @@ __device__ __forceinline__ void tpass_group(int blk, float2 *stage, Fields3D f,
     if constexpr (FAST) __syncthreads();   // every wave is done with its transform region
 #pragma unroll
     for (int c0 = 0; c0 < R; c0 += CH) {
+        const int sw = opaque_i(wv * SP + sl);
 #pragma unroll
-        for (int cc = 0; cc < CH; ++cc) stage[(cc * LINES + wv) * SP + sl] = v[c0 + cc];
+        for (int cc = 0; cc < CH; ++cc) stage[sw + cc * LINES * SP] = v[c0 + cc];
         __syncthreads();
+        const int sr = opaque_i(q * SP + row);
 #pragma unroll
         for (int cc = 0; cc < CH; ++cc) {
             const int g = lo + 64 * (c0 + cc) * plane;
-            float2 val = stage[(cc * LINES + q) * SP + row];
+            float2 val = stage[sr + cc * LINES * SP];
             if (MODE == 1) val = csub(f.Y4[g], val);
             out[g] = val;
         }
@@ -1104,13 +1118,6 @@ __device__ __forceinline__ void inv128(float2 (&v)[16], float2 *wb, int lane,
     wave_lds_sync();
     twiddle128<-1>(v, lane, tw0);
     Dft<16, -1>::run(v);
-}
-
-// An int the optimiser cannot see through: addresses derived from it are recomputed where
-// used instead of being held in registers across the kernel.
-__device__ __forceinline__ int opaque_i(int x) {
-    asm volatile("" : "+v"(x));
-    return x;
 }
 
 // row-transform output (lane: row 8 w + (lane >> 3), ky = 2 (lane & 7) + h + 16 k2) ->
