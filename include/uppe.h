@@ -103,6 +103,26 @@ int uppe3d_store_spectrum(uppe3d_plan *p, void *out_xyw, void *stream);
 int uppe3d_store_field(uppe3d_plan *p, void *out_xyt, void *stream);
 /* Device pointer of the current state (complex64, layout [omega][x][y], divided by a_scale). */
 int uppe3d_state_ptr(uppe3d_plan *p, void *stream, void **state);
+/* Copies the current state (layout [omega][x][y], divided by a_scale) to / from a caller
+ * buffer of nx*ny*nt complex64, enqueued on `stream` (no synchronisation). */
+int uppe3d_get_state(uppe3d_plan *p, void *dst, void *stream);
+int uppe3d_set_state(uppe3d_plan *p, const void *src, void *stream);
+
+/* Adjoint of the 3+1D step (reverse mode through a frozen step sequence; no counterpart in
+ * the reference).  uppe3d_set_stage_record makes every following attempt copy its stage
+ * fields X2, X3, X4 (the inputs of the 2nd..4th nonlinear evaluations, same layout as the
+ * state) into the given buffers (NULLs switch it off).  uppe3d_adjoint_step maps Bbar = dL/dB'
+ * (B' = state after an accepted step of size h from state B) to dL/dB in place, given B and
+ * the recorded X2..X4 of that step (it sets the plan's step size to h for its propagator
+ * tables).  Complex gradients: dL/dRe z + i dL/dIm z.
+ * adj_ws: uppe3d_adjoint_workspace_size bytes of device memory.  The plasma term is not
+ * differentiated: uppe3d_adjoint_check (synchronises) reports whether any line of the steps
+ * since the last check had a plasma term above FP32 resolution, and clears the flag. */
+int uppe3d_set_stage_record(uppe3d_plan *p, void *x2, void *x3, void *x4);
+int uppe3d_adjoint_workspace_size(const uppe3d_desc *d, size_t *bytes);
+int uppe3d_adjoint_step(uppe3d_plan *p, double h, const void *B, const void *X2, const void *X3,
+                        const void *X4, void *Bbar, void *adj_ws, void *stream);
+int uppe3d_adjoint_check(uppe3d_plan *p, void *stream, int *plasma_lines);
 
 /* ------------------------------------------------------------------------------------------ */
 /* 1D envelope solver (single instance and batched, one workgroup per instance)               */

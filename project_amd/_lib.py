@@ -69,6 +69,12 @@ SIGNATURES = {
     "uppe3d_store_spectrum": (c_int, [c_void_p, c_void_p, c_void_p]),
     "uppe3d_store_field": (c_int, [c_void_p, c_void_p, c_void_p]),
     "uppe3d_state_ptr": (c_int, [c_void_p, c_void_p, P(c_void_p)]),
+    "uppe3d_get_state": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "uppe3d_set_state": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "uppe3d_set_stage_record": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "uppe3d_adjoint_workspace_size": (c_int, [P(Uppe3DDesc), P(c_size_t)]),
+    "uppe3d_adjoint_step": (c_int, [c_void_p, c_double] + [c_void_p] * 7),
+    "uppe3d_adjoint_check": (c_int, [c_void_p, c_void_p, P(c_int)]),
     "uppe1d_workspace_size": (c_int, [P(Uppe1DDesc), P(c_size_t)]),
     "uppe1d_propagate": (c_int, [P(Uppe1DDesc)] + [c_void_p] * 5
                          + [c_void_p] * 5 + [P(Uppe1DRecord), c_void_p, P(Uppe1DStatus)]),

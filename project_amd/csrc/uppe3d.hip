@@ -27,6 +27,7 @@
 //   controller (one thread, FP64): the accept / reject rule of solver_3d.py:182-208.
 // Field traffic per attempt: 31 complex64 field passes (2+5+2+5+2+5+2+3+4+1).
 #include <math.h>
+#include <stddef.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
@@ -63,6 +64,7 @@ struct Ctl3D {
     unsigned int maxA_bits, maxD_bits;
     int cur, done, nonfinite, fail_count, attempts, accepted, rejected, pad_;
     int plasma_groups[8];   // per t-pass slot of the attempt: workgroups deferred to plasma
+    int adj_plasma;         // set by the adjoint t-pass when a line's plasma term matters
 };
 
 struct Tabs3D {
@@ -1246,6 +1248,211 @@ __global__ __launch_bounds__(x128::T) void xy128_kernel(Fields3D f, Tabs3D tb, C
 }
 
 // ---------------------------------------------------------------------------------------
+// Adjoint (reverse mode) of one ERK4(3)-IP step over a frozen step size h.  No counterpart in
+// the reference.  Complex gradients follow dL/dRe z + i dL/dIm z.  With the step written as
+//   P2: V = E FFT2(B), Q = E (FFT2(B) + h/6 FFT2(Y1)), X2 = iFFT2(E (FFT2(B) + h/2 FFT2(Y1)))
+//   P3: X3 = iFFT2(V + h/2 FFT2(Y2)), Q += h/3 FFT2(Y2)
+//   P4: X4 = iFFT2(E (V + h FFT2(Y3))), Q = E (Q + h/3 FFT2(Y3))
+//   P5: A4 = Q + h/6 FFT2(Y4),  B' = filter absorber iFFT2(A4)
+// (Yk = N(stage field), Y1 = N(B)), the reverse sweep is
+//   A5: Xb = fa Bb',  Yb4 = h/6 Xb,  Qb = FFT2(Xb)/Nxy
+//   A4: U = FFT2(Xb4)/Nxy, Vb = E* U, Yb3 = Nxy iFFT2(E* (h U + h/3 Qb)), Qb = E* Qb
+//   A3: U = FFT2(Xb3)/Nxy, Vb += U, Yb2 = Nxy iFFT2(h/2 U + h/3 Qb)
+//   A2: U = FFT2(Xb2)/Nxy, Yb1 = Nxy iFFT2(E* (h/2 U + h/6 Qb)),
+//       Blin = Nxy iFFT2(E* (U + Vb + Qb))
+//   TAk: Xbk = N'(Xk)^H Ybk (tadj_kernel), Bb = Blin + N'(B)^H Yb1.
+// The fifth evaluation only feeds the error estimate and has no adjoint.
+
+// LINES t-lines of src (one per wavefront) through the LDS stage: v[r] = src[omega =
+// rl + 64 r] of line lo's column.
+template <int NT, int LINES>
+__device__ __forceinline__ void line_load(float2 (&v)[NT / 64], const float2 *__restrict__ src,
+                                          float2 *stage, int lo, int plane, int tid) {
+    constexpr int R = NT / 64, CH = R < 4 ? R : 4, SP = 66;
+    const int q = tid % LINES, row = tid / LINES, wv = tid >> 6;
+    const int rl = __brev(tid & 63) >> 26;
+#pragma unroll
+    for (int c0 = 0; c0 < R; c0 += CH) {
+        float2 g[CH];
+#pragma unroll
+        for (int cc = 0; cc < CH; ++cc) g[cc] = src[lo + 64 * (c0 + cc) * plane];
+#pragma unroll
+        for (int cc = 0; cc < CH; ++cc) stage[(cc * LINES + q) * SP + row] = g[cc];
+        __syncthreads();
+#pragma unroll
+        for (int cc = 0; cc < CH; ++cc) v[c0 + cc] = stage[(cc * LINES + wv) * SP + rl];
+        __syncthreads();
+    }
+}
+
+// inverse of line_load; dst = v (+ acc when acc != nullptr)
+template <int NT, int LINES>
+__device__ __forceinline__ void line_store(const float2 (&v)[NT / 64], float2 *__restrict__ dst,
+                                           const float2 *__restrict__ acc, float2 *stage, int lo,
+                                           int plane, int tid) {
+    constexpr int R = NT / 64, CH = R < 4 ? R : 4, SP = 66;
+    const int q = tid % LINES, row = tid / LINES, wv = tid >> 6;
+    const int rl = __brev(tid & 63) >> 26;
+#pragma unroll
+    for (int c0 = 0; c0 < R; c0 += CH) {
+#pragma unroll
+        for (int cc = 0; cc < CH; ++cc) stage[(cc * LINES + wv) * SP + rl] = v[c0 + cc];
+        __syncthreads();
+#pragma unroll
+        for (int cc = 0; cc < CH; ++cc) {
+            const int g = lo + 64 * (c0 + cc) * plane;
+            float2 val = stage[(cc * LINES + q) * SP + row];
+            if (acc != nullptr) val = cadd(val, acc[g]);
+            dst[g] = val;
+        }
+        __syncthreads();
+    }
+}
+
+// Xb (+)= N'(X)^H Yb for the Kerr (+ HOKE) term of the t-pass: with x = iFFT_t(X),
+// g = iFFT_t(shock mask Yb), I = s2 |x|^2, phi = g2 I + g4 I^2, phi' = g2 + 2 g4 I and
+// gk = -i g (adjoint of the factor i):  Xb = FFT_t((phi + phi' I) gk + phi' s2 x^2 conj(gk)).
+// A line whose plasma term matters at FP32 resolution (plasma_matters) sets ctl->adj_plasma:
+// the plasma adjoint is not provided for 3D.
+template <int NT, int LINES>
+__global__ __launch_bounds__(64 * LINES) void tadj_kernel(const float2 *X, const float2 *Yb,
+                                                          float2 *out, const float2 *acc,
+                                                          Tabs3D tb, Phys3D ph, Ctl3D *ctl,
+                                                          double h, int nx, int ny) {
+    constexpr int R = NT / 64, CH = R < 4 ? R : 4, SP = 66;
+    __shared__ float2 stage[CH * LINES * SP];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int rl = __brev(lane) >> 26;
+    const int bpx = ny / LINES;
+    const int x0 = blockIdx.x / bpx, y0 = (blockIdx.x % bpx) * LINES;
+    const int plane = nx * ny;
+    const int lo = (tid / LINES) * plane + x0 * ny + y0 + tid % LINES;
+    float2 x[R], g[R];
+    line_load<NT, LINES>(x, X, stage, lo, plane, tid);
+    line_load<NT, LINES>(g, Yb, stage, lo, plane, tid);
+    const bool use_shock = ph.use_shock != 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int w = rl + 64 * r;
+        g[r] = cscale(g[r], use_shock ? tb.shock_w[w] * tb.mask_w[w] : tb.mask_w[w]);
+    }
+    wave_ifft_t<R>(x, lane, rl, tb.tw_t);
+    wave_ifft_t<R>(g, lane, rl, tb.tw_t);
+    if (plasma_matters<R>(x, ph, tb.lut, h) && lane == 0) atomicOr(&ctl->adj_plasma, 1);
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const float2 a = x[k];
+        const float I = ph.a_scale2 * cabs2(a);
+        const float phi = I * (ph.g_n2 + ph.g_n4 * I);
+        const float dphi = ph.g_n2 + 2.0f * ph.g_n4 * I;
+        const float2 gk = make_float2(g[k].y, -g[k].x);   // -i g
+        const float2 a2 = make_float2(a.x * a.x - a.y * a.y, 2.0f * a.x * a.y);
+        const float c = phi + dphi * I, d = dphi * ph.a_scale2;
+        // c gk + d a^2 conj(gk)
+        x[k] = make_float2(c * gk.x + d * (a2.x * gk.x + a2.y * gk.y),
+                           c * gk.y + d * (a2.y * gk.x - a2.x * gk.y));
+    }
+    wave_fft_t<R>(x, lane, rl, tb.tw_t);
+    line_store<NT, LINES>(x, out, acc, stage, lo, plane, tid);
+}
+
+struct Adj3D {
+    float2 *Bb, *Vb, *Qb, *Yb, *Xb, *Blin;
+};
+
+// The xy part of the reverse sweep (MODE 5, 4, 3, 2 as A5 .. A2 above), one workgroup per
+// omega plane, plane in LDS, natural layout for the adjoint k-space fields Vb and Qb.
+// E = exp(h/2 D) from the per-plane tables of the step's forward replay.
+template <int NX, int NY, int MODE>
+__global__ __launch_bounds__(xy_threads(NX, NY)) void xadj_kernel(Adj3D a, Tabs3D tb, float h) {
+    using C = XY<NX, NY>;
+    __shared__ float2 lds[NX * C::LD];
+    const int w = blockIdx.x, tid = threadIdx.x;
+    const size_t pb = (size_t)w * C::P;
+    const float inv = 1.0f / (float)C::P;
+    const float2 ed = tb.ph_d[w], eref = tb.ph_r[w];
+    float2 *const Vb = a.Vb + pb, *const Qb = a.Qb + pb, *const Yb = a.Yb + pb;
+    if (MODE == 5) {
+        const float fw = tb.filt_w[w];
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) {
+            const int e = tid + C::T * i;
+            const float2 val = cscale(a.Bb[pb + e], fw * expf(-tb.damp[e] * h));
+            Yb[e] = cscale(val, h / 6.0f);
+            lds[lslot<NY>(e)] = val;
+        }
+        __syncthreads();
+        fft2_lds<NX, NY, 1>(lds, tid, tb);
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) {
+            const int e = tid + C::T * i;
+            Qb[e] = cscale(lds[lslot<NY>(e)], inv);
+        }
+        return;
+    }
+    const float2 *const Xb = a.Xb + pb;
+#pragma unroll
+    for (int i = 0; i < C::EPT; ++i) {
+        const int e = tid + C::T * i;
+        lds[lslot<NY>(e)] = Xb[e];
+    }
+    __syncthreads();
+    fft2_lds<NX, NY, 1>(lds, tid, tb);
+    float2 ab[MODE == 2 ? C::EPT : 1];
+#pragma unroll
+    for (int i = 0; i < C::EPT; ++i) {
+        const int e = tid + C::T * i;
+        const float2 U = cscale(lds[lslot<NY>(e)], inv);
+        const float2 q = Qb[e];
+        if (MODE == 4) {
+            const float2 E = eprop<NY>(e, w, 0.5f * h, tb, ed, eref);
+            Vb[e] = cmulc(U, E);
+            Qb[e] = cmulc(q, E);
+            lds[lslot<NY>(e)] = cmulc(caxpy(cscale(U, h), h / 3.0f, q), E);
+        } else if (MODE == 3) {
+            Vb[e] = cadd(Vb[e], U);
+            lds[lslot<NY>(e)] = caxpy(cscale(U, 0.5f * h), h / 3.0f, q);
+        } else {  // MODE 2
+            const float2 E = eprop<NY>(e, w, 0.5f * h, tb, ed, eref);
+            ab[MODE == 2 ? i : 0] = cmulc(cadd(cadd(U, Vb[e]), q), E);
+            lds[lslot<NY>(e)] = cmulc(caxpy(cscale(U, 0.5f * h), h / 6.0f, q), E);
+        }
+    }
+    __syncthreads();
+    fft2_lds<NX, NY, -1>(lds, tid, tb);
+#pragma unroll
+    for (int i = 0; i < C::EPT; ++i) {
+        const int e = tid + C::T * i;
+        Yb[e] = lds[lslot<NY>(e)];
+    }
+    if (MODE == 2) {
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) lds[lslot<NY>(tid + C::T * i)] = ab[MODE == 2 ? i : 0];
+        __syncthreads();
+        fft2_lds<NX, NY, -1>(lds, tid, tb);
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) {
+            const int e = tid + C::T * i;
+            a.Blin[pb + e] = lds[lslot<NY>(e)];
+        }
+    }
+}
+
+__global__ void adj_h_kernel(Ctl3D *c, double h) { c->h = h; }
+
+// copy between a caller buffer and the current state (ctl->cur read on the device)
+__global__ void state_copy_kernel(const Ctl3D *ctl, Fields3D f, float2 *ext, size_t n,
+                                  int to_state) {
+    float2 *st = ctl->cur ? f.B1 : f.B0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        if (to_state) st[i] = ext[i];
+        else ext[i] = st[i];
+    }
+}
+
+// ---------------------------------------------------------------------------------------
 // Step controller (solver_3d.py:182-208), one thread, FP64.  Also applies the top-of-loop
 // clip h = min(h, L - z) of the next iteration and the h < 1e-15 stop.
 __global__ void ctl_kernel(Ctl3D *c) {
@@ -1346,6 +1553,7 @@ struct uppe3d_plan {
     int *list;    // t-pass workgroups deferred to the plasma instantiation
     int ncu;      // compute units of the device (grid of the plasma launch)
     size_t ws_bytes;
+    void *rec[3]; // optional copies of the stage fields X2, X3, X4 of every attempt
 };
 
 namespace {
@@ -1494,6 +1702,14 @@ int launch_xy(uppe3d_plan *p, int mode, hipStream_t s) {
         if (r_ != UPPE_OK) return r_; \
     } while (0)
 
+size_t field_bytes(const uppe3d_desc &d) { return sizeof(float2) * (size_t)d.nx * d.ny * d.nt; }
+
+int record_stage(uppe3d_plan *p, int k, hipStream_t s) {
+    if (p->rec[k] != nullptr)
+        UPPE_HIP(hipMemcpyAsync(p->rec[k], p->f.X, field_bytes(p->d), hipMemcpyDeviceToDevice, s));
+    return UPPE_OK;
+}
+
 // One attempt: 11 launches.  With `ev` (12 events) an event is recorded before the first
 // and after every launch, so the caller can read per-kernel durations.
 int enqueue_attempt(uppe3d_plan *p, hipStream_t s, hipEvent_t *ev = nullptr) {
@@ -1507,14 +1723,17 @@ int enqueue_attempt(uppe3d_plan *p, hipStream_t s, hipEvent_t *ev = nullptr) {
     RET(launch_tpass(p, 0, 0, nullptr, 1, f.Yw, s));   // T1 on the state
     RET(mark());
     RET(launch_xy(p, 2, s));                          // P2
+    RET(record_stage(p, 0, s));
     RET(mark());
     RET(launch_tpass(p, 0, 1, f.X, 0, f.Yw, s));        // T2
     RET(mark());
     RET(launch_xy(p, 3, s));                          // P3
+    RET(record_stage(p, 1, s));
     RET(mark());
     RET(launch_tpass(p, 0, 2, f.X, 0, f.Yw, s));        // T3
     RET(mark());
     RET(launch_xy(p, 4, s));                          // P4
+    RET(record_stage(p, 2, s));
     RET(mark());
     RET(launch_tpass(p, 0, 3, f.X, 0, f.Y4, s));        // T4
     RET(mark());
@@ -1628,6 +1847,7 @@ int uppe3d_plan_create(const uppe3d_desc *d, void *workspace, size_t workspace_b
     p->ws_bytes = L.total;
     p->ctl = (Ctl3D *)(ws + L.ctl);
     p->list = (int *)(ws + L.list);
+    p->rec[0] = p->rec[1] = p->rec[2] = nullptr;
     {
         int dev = 0;
         UPPE_HIP(hipGetDevice(&dev));
@@ -1872,6 +2092,133 @@ int uppe3d_state_ptr(uppe3d_plan *p, void *stream, void **state) {
     uppe3d_status st;
     RET(uppe3d_status_get(p, stream, &st));
     *state = st.cur ? p->f.B1 : p->f.B0;
+    return UPPE_OK;
+}
+
+int uppe3d_get_state(uppe3d_plan *p, void *dst, void *stream) {
+    UPPE_CHECK(p && dst, "uppe3d_get_state: null argument");
+    const size_t n = (size_t)p->d.nx * p->d.ny * p->d.nt;
+    state_copy_kernel<<<1024, 256, 0, (hipStream_t)stream>>>(p->ctl, p->f, (float2 *)dst, n, 0);
+    UPPE_HIP(hipGetLastError());
+    return UPPE_OK;
+}
+
+int uppe3d_set_state(uppe3d_plan *p, const void *src, void *stream) {
+    UPPE_CHECK(p && src, "uppe3d_set_state: null argument");
+    const size_t n = (size_t)p->d.nx * p->d.ny * p->d.nt;
+    state_copy_kernel<<<1024, 256, 0, (hipStream_t)stream>>>(p->ctl, p->f, (float2 *)src, n, 1);
+    UPPE_HIP(hipGetLastError());
+    return UPPE_OK;
+}
+
+int uppe3d_set_stage_record(uppe3d_plan *p, void *x2, void *x3, void *x4) {
+    UPPE_CHECK(p != nullptr, "uppe3d_set_stage_record: null plan");
+    UPPE_CHECK((x2 && x3 && x4) || (!x2 && !x3 && !x4),
+               "uppe3d_set_stage_record: give all three buffers or none");
+    p->rec[0] = x2;
+    p->rec[1] = x3;
+    p->rec[2] = x4;
+    return UPPE_OK;
+}
+
+int uppe3d_adjoint_workspace_size(const uppe3d_desc *d, size_t *bytes) {
+    RET(check_desc(d));
+    UPPE_CHECK(bytes != nullptr, "uppe3d_adjoint_workspace_size: null output");
+    *bytes = 5 * align_up(field_bytes(*d));
+    return UPPE_OK;
+}
+
+}  // extern "C"
+
+namespace {
+template <int N>
+void launch_xadj(int mode, int nt, const Adj3D &a, const Tabs3D &tb, float h, hipStream_t s) {
+    const dim3 grid(nt), block(XY<N, N>::T);
+    switch (mode) {
+        case 5: xadj_kernel<N, N, 5><<<grid, block, 0, s>>>(a, tb, h); break;
+        case 4: xadj_kernel<N, N, 4><<<grid, block, 0, s>>>(a, tb, h); break;
+        case 3: xadj_kernel<N, N, 3><<<grid, block, 0, s>>>(a, tb, h); break;
+        default: xadj_kernel<N, N, 2><<<grid, block, 0, s>>>(a, tb, h); break;
+    }
+}
+
+int xadj(uppe3d_plan *p, int mode, const Adj3D &a, float h, hipStream_t s) {
+    switch (p->d.nx) {
+        case 16: launch_xadj<16>(mode, p->d.nt, a, p->tb, h, s); break;
+        case 32: launch_xadj<32>(mode, p->d.nt, a, p->tb, h, s); break;
+        case 64: launch_xadj<64>(mode, p->d.nt, a, p->tb, h, s); break;
+        case 128: launch_xadj<128>(mode, p->d.nt, a, p->tb, h, s); break;
+        default: return UPPE_EINVAL;
+    }
+    UPPE_HIP(hipGetLastError());
+    return UPPE_OK;
+}
+
+#define LAUNCH_TA(NT, LINES)                                                                 \
+    case NT:                                                                                 \
+        tadj_kernel<NT, LINES><<<p->d.nx * (p->d.ny / LINES), 64 * LINES, 0, s>>>(           \
+            X, Yb, out, acc, p->tb, p->ph, p->ctl, h, p->d.nx, p->d.ny);                     \
+        break;
+
+int tadj(uppe3d_plan *p, const float2 *X, const float2 *Yb, float2 *out, const float2 *acc,
+         double h, hipStream_t s) {
+    switch (p->d.nt) {
+        LAUNCH_TA(128, 16)
+        LAUNCH_TA(256, 16)
+        LAUNCH_TA(512, 16)
+        LAUNCH_TA(1024, 16)
+        LAUNCH_TA(2048, 8)
+        default:
+            return UPPE_EINVAL;
+    }
+    UPPE_HIP(hipGetLastError());
+    return UPPE_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int uppe3d_adjoint_step(uppe3d_plan *p, double h, const void *B, const void *X2, const void *X3,
+                        const void *X4, void *Bbar, void *adj_ws, void *stream) {
+    UPPE_CHECK(p && B && X2 && X3 && X4 && Bbar && adj_ws && h > 0.0,
+               "uppe3d_adjoint_step: bad argument");
+    UPPE_CHECK(((uintptr_t)adj_ws % kAlign) == 0, "uppe3d_adjoint_step: workspace not 256-B aligned");
+    hipStream_t s = (hipStream_t)stream;
+    const size_t fb = align_up(field_bytes(p->d));
+    char *w = (char *)adj_ws;
+    Adj3D a;
+    a.Bb = (float2 *)Bbar;
+    a.Vb = (float2 *)(w);
+    a.Qb = (float2 *)(w + fb);
+    a.Yb = (float2 *)(w + 2 * fb);
+    a.Xb = (float2 *)(w + 3 * fb);
+    a.Blin = (float2 *)(w + 4 * fb);
+    const float hf = (float)h;
+    // propagator tables of this step size
+    adj_h_kernel<<<1, 1, 0, s>>>(p->ctl, h);
+    phase_kernel<<<(p->d.nt + 255) / 256, 256, 0, s>>>(p->ctl, p->tb, p->d.nt);
+    UPPE_HIP(hipGetLastError());
+    RET(xadj(p, 5, a, hf, s));
+    RET(tadj(p, (const float2 *)X4, a.Yb, a.Xb, nullptr, h, s));
+    RET(xadj(p, 4, a, hf, s));
+    RET(tadj(p, (const float2 *)X3, a.Yb, a.Xb, nullptr, h, s));
+    RET(xadj(p, 3, a, hf, s));
+    RET(tadj(p, (const float2 *)X2, a.Yb, a.Xb, nullptr, h, s));
+    RET(xadj(p, 2, a, hf, s));
+    RET(tadj(p, (const float2 *)B, a.Yb, a.Bb, a.Blin, h, s));
+    return UPPE_OK;
+}
+
+int uppe3d_adjoint_check(uppe3d_plan *p, void *stream, int *plasma_lines) {
+    UPPE_CHECK(p && plasma_lines, "uppe3d_adjoint_check: null argument");
+    hipStream_t s = (hipStream_t)stream;
+    UPPE_HIP(hipStreamSynchronize(s));
+    Ctl3D c;
+    UPPE_HIP(hipMemcpy(&c, p->ctl, sizeof c, hipMemcpyDeviceToHost));
+    *plasma_lines = c.adj_plasma;
+    const int zero = 0;
+    UPPE_HIP(hipMemcpy((char *)p->ctl + offsetof(Ctl3D, adj_plasma), &zero, sizeof zero,
+                       hipMemcpyHostToDevice));
     return UPPE_OK;
 }
 
