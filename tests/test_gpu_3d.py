@@ -39,6 +39,8 @@ CASES = [
     # scan; peak 5e18 W/m^2 makes W dt > exp(gdt) and takes the serial recurrence
     ("32x256 plasma scan", 32, 256, 4e-3, False, False, 0.005, 1e-5, 2e-5),
     ("32x256 plasma stress", 32, 256, 30e-3, False, True, 0.001, 1e-4, 1e-3),
+    # the longest supported lines (NT = 2048, 32 values per lane)
+    ("16x2048 kerr+hoke", 16, 2048, 0.8e-3, True, False, 0.02, 1e-5, 2e-5),
 ]
 
 
