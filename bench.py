@@ -33,6 +33,18 @@ GROUPS = {"tpass": ["T1", "T2", "T3", "T4"], "tpass_final": ["T5"], "xy_stage": 
           "xy_stage_noexp": ["P3"], "xy_final": ["P5"], "xy_err": ["P6"], "ctl": ["CTL"]}
 
 
+def measured_traffic(nx, nt):
+    """HBM bytes per t-pass launch (FETCH_SIZE + WRITE_SIZE, calibrated against a copy kernel)
+    from the committed rocprofv3 counter summary of this grid, or None."""
+    path = os.path.join(ROOT, "profiles", "uppe3d_%dx%dx%d_hbm_traffic.md" % (nx, nx, nt))
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("tpass_bytes_per_launch ="):
+            return int(line.split("=")[1])
+    return None
+
+
 def cpu_baseline(nx, nt, attempts, workers):
     """FP64 oracle (NumPy/SciPy restatement of the reference) on the same workload."""
     from oracle import uppe3d as o3
@@ -274,9 +286,13 @@ def main():
     dom = max((g for g in kernels if kernels[g]["bytes_per_launch"] > 0),
               key=lambda g: kernels[g]["share"])
     d = kernels[dom]
+    traffic = measured_traffic(nx, nt) if dom == "tpass" else None
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(d["GBps"], 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(d["GBps"] / HBM_PEAK_GBS, 4),
-                "traffic": None}
+                "traffic": traffic}
+    if traffic is not None:
+        roofline["traffic_source"] = ("profiles/uppe3d_%dx%dx%d_hbm_traffic.md (rocprofv3 "
+                                      "FETCH_SIZE + WRITE_SIZE, copy-calibrated)" % (nx, nx, nt))
     step_gbs_37 = 37 * field_bytes * steps_per_s / 1e9
     step_gbs_31 = 31 * field_bytes * steps_per_s / 1e9
 
