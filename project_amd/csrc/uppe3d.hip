@@ -1453,6 +1453,163 @@ __global__ void state_copy_kernel(const Ctl3D *ctl, Fields3D f, float2 *ext, siz
 }
 
 // ---------------------------------------------------------------------------------------
+// Planes larger than one workgroup's LDS (256 x 256, 512 KB): the 2D transform is split into
+// a column pass (length-256 transforms along x, stride Ny) and a row kernel (transform along
+// y, the stage arithmetic of one xy pass in (kx, ky), inverse transform along y); a second
+// column pass finishes the inverse.  Intermediates are written in place into fields that are
+// free at that point of the step.  Every length-256 transform runs in a wave-private LDS line
+// (LineFFT, Stockham, natural order in and out).  The k-space fields V, Q (and the adjoint
+// ones) are in natural (kx, ky) order.
+
+// Column transforms, src -> dst (may alias): for every plane and column y the length-NX
+// transform along x; a workgroup takes 16 columns of one plane, whose HBM rows are 16
+// consecutive y (128 bytes).  DIR = +1 forward, -1 inverse, both unnormalised; the output is
+// scaled by `scale`.  src_state: read the current state instead of src.
+template <int NX, int DIR>
+__global__ __launch_bounds__(1024) void colpass_kernel(const float2 *src, float2 *dst, int ny,
+                                                       float scale, const float2 *tw, Fields3D f,
+                                                       const Ctl3D *ctl, int src_state,
+                                                       int check_done) {
+    constexpr int EPT = NX / 64, LP = NX + NX / 16;
+    __shared__ float2 lines[16 * LP];
+    if (check_done && ctl->done) return;
+    if (src_state) src = ctl->cur ? f.B1 : f.B0;
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int cpp = ny / 16;
+    const int w = blockIdx.x / cpp, y0 = (blockIdx.x % cpp) * 16;
+    const size_t base = (size_t)w * NX * ny + y0;
+    const int q = tid & 15, row = tid >> 4;
+#pragma unroll
+    for (int c = 0; c < EPT; ++c) {
+        const int x = row + 64 * c;
+        lines[q * LP + pidx(x)] = src[base + (size_t)x * ny + q];
+    }
+    __syncthreads();
+    LineFFT<NX, EPT, 1, DIR>::run(lines + wv * LP, lane, tw);
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < EPT; ++c) {
+        const int x = row + 64 * c;
+        dst[base + (size_t)x * ny + q] = cscale(lines[q * LP + pidx(x)], scale);
+    }
+}
+
+// One row (x fixed) of one plane per wavefront, 16 rows per workgroup.  MODE 2..6: the xy
+// passes of xypass_kernel (inputs already transformed along x); MODE 22..25: the adjoint
+// passes A2..A5 of xadj_kernel.
+template <int NY, int MODE>
+__global__ __launch_bounds__(1024) void row_kernel(Fields3D f, Adj3D a, Tabs3D tb, Ctl3D *ctl,
+                                                   float h_adj, int nx) {
+    constexpr int EPT = NY / 64, LP = NY + NY / 16;
+    constexpr bool ADJ = MODE >= 20;
+    __shared__ float2 lines[16 * LP];
+    __shared__ unsigned int red[16];
+    if (!ADJ && ctl->done) return;
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int rpp = nx / 16;
+    const int w = blockIdx.x / rpp, x = (blockIdx.x % rpp) * 16 + wv;
+    const size_t rb = ((size_t)w * nx + x) * NY;
+    float2 *line = lines + wv * LP;
+    const float h = ADJ ? h_adj : (float)ctl->h;
+    const float2 ed = tb.ph_d[w], eref = tb.ph_r[w];
+    const float kxx = tb.kx2[x];
+    const float inv = 1.0f / ((float)nx * (float)NY);
+    // forward transform of a row of src into v (v[r] = k_y = lane + 64 r)
+    auto fwd = [&](const float2 *src, float2 (&v)[EPT]) {
+#pragma unroll
+        for (int r = 0; r < EPT; ++r) line[pidx(lane + 64 * r)] = src[rb + lane + 64 * r];
+        wave_lds_sync();
+        LineFFT<NY, EPT, 1, 1>::run(line, lane, tb.tw_y);
+#pragma unroll
+        for (int r = 0; r < EPT; ++r) v[r] = line[pidx(lane + 64 * r)];
+        wave_lds_sync();
+    };
+    auto inv_store = [&](const float2 (&v)[EPT], float2 *dst, float scale) {
+#pragma unroll
+        for (int r = 0; r < EPT; ++r) line[pidx(lane + 64 * r)] = v[r];
+        wave_lds_sync();
+        LineFFT<NY, EPT, 1, -1>::run(line, lane, tb.tw_y);
+#pragma unroll
+        for (int r = 0; r < EPT; ++r) dst[rb + lane + 64 * r] = cscale(line[pidx(lane + 64 * r)], scale);
+        wave_lds_sync();
+    };
+    auto E_at = [&](int r) {
+        return eprop_k(kxx + tb.ky2[lane + 64 * r], w, 0.5f * h, tb, ed, eref);
+    };
+    float2 b[EPT], o[EPT];
+    const float2 *in = (MODE == 5) ? f.X : (ADJ ? a.Xb : f.Yw);
+    fwd(in, b);
+    unsigned int m = 0;
+#pragma unroll
+    for (int r = 0; r < EPT; ++r) {
+        const size_t e = rb + lane + 64 * r;
+        if (MODE == 3) {
+            o[r] = caxpy(f.V[e], 0.5f * h, b[r]);
+            f.Q[e] = caxpy(f.Q[e], h / 3.0f, b[r]);
+        } else if (MODE == 4) {
+            const float2 E = E_at(r);
+            o[r] = cmul(E, caxpy(f.V[e], h, b[r]));
+            f.Q[e] = cmul(E, caxpy(f.Q[e], h / 3.0f, b[r]));
+        } else if (MODE == 5) {
+            o[r] = caxpy(f.Q[e], h / 6.0f, b[r]);
+            const unsigned int u = __float_as_uint(cabs2(o[r]));
+            m = u > m ? u : m;
+        } else if (MODE == 6) {
+            const unsigned int u = __float_as_uint(cabs2(b[r]));
+            m = u > m ? u : m;
+        } else if (MODE == 25) {
+            a.Qb[e] = cscale(b[r], inv);
+        } else if (MODE == 24) {
+            const float2 U = cscale(b[r], inv), q = a.Qb[e], E = E_at(r);
+            a.Vb[e] = cmulc(U, E);
+            a.Qb[e] = cmulc(q, E);
+            o[r] = cmulc(caxpy(cscale(U, h), h / 3.0f, q), E);
+        } else if (MODE == 23) {
+            const float2 U = cscale(b[r], inv);
+            a.Vb[e] = cadd(a.Vb[e], U);
+            o[r] = caxpy(cscale(U, 0.5f * h), h / 3.0f, a.Qb[e]);
+        }
+    }
+    if (MODE == 2) {
+        float2 av[EPT];
+        fwd(f.X, av);   // state, transformed along x
+#pragma unroll
+        for (int r = 0; r < EPT; ++r) {
+            const size_t e = rb + lane + 64 * r;
+            const float2 E = E_at(r);
+            f.V[e] = cmul(E, av[r]);
+            f.Q[e] = cmul(E, caxpy(av[r], h / 6.0f, b[r]));
+            o[r] = cmul(E, caxpy(av[r], 0.5f * h, b[r]));
+        }
+    }
+    if (MODE == 22) {
+        float2 ab[EPT];
+#pragma unroll
+        for (int r = 0; r < EPT; ++r) {
+            const size_t e = rb + lane + 64 * r;
+            const float2 U = cscale(b[r], inv), q = a.Qb[e], E = E_at(r);
+            ab[r] = cmulc(cadd(cadd(U, a.Vb[e]), q), E);
+            o[r] = cmulc(caxpy(cscale(U, 0.5f * h), h / 6.0f, q), E);
+        }
+        inv_store(ab, a.Blin, 1.0f);
+    }
+    if (MODE == 5 || MODE == 6) block_max_to<1024>(m, MODE == 5 ? &ctl->maxA_bits : &ctl->maxD_bits, red, tid);
+    if (MODE == 2 || MODE == 3 || MODE == 4 || MODE == 5) inv_store(o, f.X, 1.0f / (float)NY);
+    if (MODE == 22 || MODE == 23 || MODE == 24) inv_store(o, a.Yb, 1.0f);
+}
+
+// adjoint A5 prelude: Xb = filter absorber Bb', Yb4 = h/6 Xb
+__global__ void a5pre_kernel(Adj3D a, Tabs3D tb, float h, int plane, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int w = (int)(i / plane), e = (int)(i % plane);
+        const float2 val = cscale(a.Bb[i], tb.filt_w[w] * expf(-tb.damp[e] * h));
+        a.Xb[i] = val;
+        a.Yb[i] = cscale(val, h / 6.0f);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
 // Step controller (solver_3d.py:182-208), one thread, FP64.  Also applies the top-of-loop
 // clip h = min(h, L - z) of the next iteration and the h < 1e-15 stop.
 __global__ void ctl_kernel(Ctl3D *c) {
@@ -1609,8 +1766,8 @@ bool pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 int check_desc(const uppe3d_desc *d) {
     UPPE_CHECK(d != nullptr, "uppe3d: null descriptor");
     UPPE_CHECK(d->nx == d->ny, "uppe3d: nx (%d) must equal ny (%d)", d->nx, d->ny);
-    UPPE_CHECK(d->nx == 16 || d->nx == 32 || d->nx == 64 || d->nx == 128,
-               "uppe3d: nx = %d unsupported (16, 32, 64, 128)", d->nx);
+    UPPE_CHECK(d->nx == 16 || d->nx == 32 || d->nx == 64 || d->nx == 128 || d->nx == 256,
+               "uppe3d: nx = %d unsupported (16, 32, 64, 128, 256)", d->nx);
     UPPE_CHECK(pow2(d->nt) && d->nt >= 128 && d->nt <= 2048,
                "uppe3d: nt = %d unsupported (power of two in [128, 2048])", d->nt);
     UPPE_CHECK(d->lut_n >= 2, "uppe3d: lut_n must be >= 2");
@@ -1680,6 +1837,51 @@ int launch_xy(uppe3d_plan *p, int mode, hipStream_t s) {
         case 16: return launch_xy_n<16>(p, mode, s);
         case 32: return launch_xy_n<32>(p, mode, s);
         case 64: return launch_xy_n<64>(p, mode, s);
+        case 256: {
+            // column pass -> row kernel -> inverse column pass (planes beyond LDS)
+            const int nt = p->d.nt, ny = p->d.ny;
+            const dim3 gc(nt * (ny / 16)), gr(nt * (p->d.nx / 16)), blk(1024);
+            Fields3D &f = p->f;
+            const Adj3D none = {};
+            const float inx = 1.0f / (float)p->d.nx;
+            auto colF = [&](const float2 *src, float2 *dst, int state) {
+                colpass_kernel<256, 1><<<gc, blk, 0, s>>>(src, dst, ny, 1.0f, p->tb.tw_x, f,
+                                                          p->ctl, state, 1);
+            };
+            auto colI = [&](float2 *buf) {
+                colpass_kernel<256, -1><<<gc, blk, 0, s>>>(buf, buf, ny, inx, p->tb.tw_x, f,
+                                                           p->ctl, 0, 1);
+            };
+            switch (mode) {
+                case 2:
+                    colF(nullptr, f.X, 1);
+                    colF(f.Yw, f.Yw, 0);
+                    row_kernel<256, 2><<<gr, blk, 0, s>>>(f, none, p->tb, p->ctl, 0.0f, p->d.nx);
+                    colI(f.X);
+                    break;
+                case 3:
+                    colF(f.Yw, f.Yw, 0);
+                    row_kernel<256, 3><<<gr, blk, 0, s>>>(f, none, p->tb, p->ctl, 0.0f, p->d.nx);
+                    colI(f.X);
+                    break;
+                case 4:
+                    colF(f.Yw, f.Yw, 0);
+                    row_kernel<256, 4><<<gr, blk, 0, s>>>(f, none, p->tb, p->ctl, 0.0f, p->d.nx);
+                    colI(f.X);
+                    break;
+                case 5:
+                    colF(f.Y4, f.X, 0);
+                    row_kernel<256, 5><<<gr, blk, 0, s>>>(f, none, p->tb, p->ctl, 0.0f, p->d.nx);
+                    colI(f.X);
+                    break;
+                default:
+                    colF(f.Yw, f.Yw, 0);
+                    row_kernel<256, 6><<<gr, blk, 0, s>>>(f, none, p->tb, p->ctl, 0.0f, p->d.nx);
+                    break;
+            }
+            UPPE_HIP(hipGetLastError());
+            return UPPE_OK;
+        }
         case 128: {
             const dim3 grid(p->d.nt), block(x128::T);
             switch (mode) {
@@ -2142,8 +2344,47 @@ void launch_xadj(int mode, int nt, const Adj3D &a, const Tabs3D &tb, float h, hi
     }
 }
 
+int xadj256(uppe3d_plan *p, int mode, const Adj3D &a, float h, hipStream_t s) {
+    const int nt = p->d.nt, ny = p->d.ny, nx = p->d.nx;
+    const dim3 gc(nt * (ny / 16)), gr(nt * (nx / 16)), blk(1024);
+    auto col = [&](float2 *buf, int dir) {
+        if (dir > 0)
+            colpass_kernel<256, 1><<<gc, blk, 0, s>>>(buf, buf, ny, 1.0f, p->tb.tw_x, p->f,
+                                                      p->ctl, 0, 0);
+        else
+            colpass_kernel<256, -1><<<gc, blk, 0, s>>>(buf, buf, ny, 1.0f, p->tb.tw_x, p->f,
+                                                       p->ctl, 0, 0);
+    };
+    switch (mode) {
+        case 5:
+            a5pre_kernel<<<2048, 256, 0, s>>>(a, p->tb, h, nx * ny, (size_t)nx * ny * nt);
+            col(a.Xb, 1);
+            row_kernel<256, 25><<<gr, blk, 0, s>>>(p->f, a, p->tb, p->ctl, h, nx);
+            break;
+        case 4:
+            col(a.Xb, 1);
+            row_kernel<256, 24><<<gr, blk, 0, s>>>(p->f, a, p->tb, p->ctl, h, nx);
+            col(a.Yb, -1);
+            break;
+        case 3:
+            col(a.Xb, 1);
+            row_kernel<256, 23><<<gr, blk, 0, s>>>(p->f, a, p->tb, p->ctl, h, nx);
+            col(a.Yb, -1);
+            break;
+        default:
+            col(a.Xb, 1);
+            row_kernel<256, 22><<<gr, blk, 0, s>>>(p->f, a, p->tb, p->ctl, h, nx);
+            col(a.Yb, -1);
+            col(a.Blin, -1);
+            break;
+    }
+    UPPE_HIP(hipGetLastError());
+    return UPPE_OK;
+}
+
 int xadj(uppe3d_plan *p, int mode, const Adj3D &a, float h, hipStream_t s) {
     switch (p->d.nx) {
+        case 256: return xadj256(p, mode, a, h, s);
         case 16: launch_xadj<16>(mode, p->d.nt, a, p->tb, h, s); break;
         case 32: launch_xadj<32>(mode, p->d.nt, a, p->tb, h, s); break;
         case 64: launch_xadj<64>(mode, p->d.nt, a, p->tb, h, s); break;
