@@ -169,7 +169,7 @@ def bench_adjoint():
             "adjoint_over_forward": round(b / f, 3)}
 
 
-def bench_adjoint3d(nx=128, nt=2048, L=0.3):
+def bench_adjoint3d(nx=256, nt=2048, L=0.3):
     """3+1D adjoint: gradient of the on-axis spectral energy in 600-700 nm with respect to the
     input spatial phase (nx^2) and spectral phase (nt), argon (config 1 physics), 0.8 mJ,
     over L with sqrt(M) checkpoints.  Frozen forward and backward timed separately."""
