@@ -97,6 +97,19 @@ __device__ float2 *lds_fft(float2 *a, float2 *b, int n, const float2 *__restrict
 }
 
 template <int T>
+__device__ double block_max(double v, double *red, int tid) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    __syncthreads();
+    if ((tid & 63) == 0) red[tid >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < T / 64; ++w) s = fmax(s, red[w]);
+    return s;
+}
+
+template <int T>
 __device__ double block_sum(double v, double *red, int tid) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -205,21 +218,55 @@ __device__ void plasma_coeffs_1d(float2 *buf, int n, const Inst1D &P, double *sc
     __syncthreads();
 }
 
+// Fills buf[t] = (W(t), I(t)) for the time samples a (element e = tid + T i) and decides
+// whether the plasma term is evaluated: not when every rate is zero, nor when its
+// whole-propagation bound is below one FP32 ulp of the field.  Bound: with W = sigma I^K
+// (K >= 1) every sample has W <= W(I_max) and W / I <= W(I_max) / I_max, and the recurrence
+// gives rho <= f rho_nt sum_t W, so |c1|, |c2| <= Ui rho_nt W(I_max) / (2 I_max) +
+// sigma/2 rho_max (1 + omega0 tau_c); the term is left out when L times that is below 2^-24
+// (avalanche off).  The forward and the adjoint take the same decision from the same field.
+template <int T, int E>
+__device__ bool plasma_setup(const Inst1D &P, const float2 (&a)[E], float2 *buf, double *scan,
+                             int tid) {
+    double wsum = 0.0, imax = 0.0;
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+        const double I = (double)cabs2(a[i]) * P.inv_area;
+        double pw = 1.0;   // I^K by repeated multiplication (integer K)
+        for (int k = 0; k < P.K; ++k) pw *= I;
+        const float w = (float)(P.sigma_K * pw);
+        buf[tid + T * i] = make_float2(w, (float)I);
+        wsum += (double)w;
+        imax = fmax(imax, I);
+    }
+    wsum = block_sum<T>(wsum, scan, tid);
+    imax = block_max<T>(imax, scan + 32, tid);
+    if (!(wsum > 0.0)) return false;
+    if (P.avalanche) return true;
+    double pw = 1.0;
+    for (int k = 0; k < P.K; ++k) pw *= imax;
+    const double wmax = (double)(float)(P.sigma_K * pw);
+    const double rmax = P.rho_nt * fmin(1.0, P.f_rec * wsum);
+    const double cb = P.Ui * P.rho_nt * wmax / (2.0 * imax + 1e-30) +
+                      P.half_sigma * rmax * (1.0 + P.w0tc);
+    return !(P.L * cb < 5.9604644775390625e-8);
+}
+
 // N(F): F in omega (registers, e = tid + T i) -> out in omega.  If snap != nullptr the time
 // samples of F are also written there (history slot).
-template <int T, int NMAX>
-__device__ void neval(const Inst1D &P, const float2 (&F)[MAXE], float2 (&out)[MAXE], float2 *bufA,
+template <int T, int NMAX, int E>
+__device__ void neval(const Inst1D &P, const float2 (&F)[E], float2 (&out)[E], float2 *bufA,
                       float2 *bufB, double *scan, int tid, float2 *snap) {
-    const int n = P.nt, E = n / T;
+    constexpr int n = T * E;
 #pragma unroll
-    for (int i = 0; i < MAXE; ++i)
+    for (int i = 0; i < E; ++i)
         if (i < E) bufA[tid + T * i] = F[i];
     __syncthreads();
     float2 *r = lds_fft<T, -1>(bufA, bufB, n, P.tw, tid);
     const float invn = 1.0f / (float)n;
-    float2 a[MAXE], Ie[MAXE];
+    float2 a[E], Ie[E];
 #pragma unroll
-    for (int i = 0; i < MAXE; ++i)
+    for (int i = 0; i < E; ++i)
         if (i < E) {
             a[i] = cscale(r[tid + T * i], invn);
             Ie[i] = make_float2(cabs2(a[i]), 0.0f);
@@ -229,20 +276,20 @@ __device__ void neval(const Inst1D &P, const float2 (&F)[MAXE], float2 (&out)[MA
     if (P.use_raman) {
         // I_eff = (1 - f_R) I + f_R ifft(fft(I) H)   (complex, erk43ip_method.py:206-217)
 #pragma unroll
-        for (int i = 0; i < MAXE; ++i)
+        for (int i = 0; i < E; ++i)
             if (i < E) bufA[tid + T * i] = Ie[i];
         __syncthreads();
         float2 *q = lds_fft<T, 1>(bufA, bufB, n, P.tw, tid);
         float2 *o = (q == bufA) ? bufB : bufA;
 #pragma unroll
-        for (int i = 0; i < MAXE; ++i)
+        for (int i = 0; i < E; ++i)
             if (i < E) o[tid + T * i] = cmul(q[tid + T * i], P.H[tid + T * i]);
         __syncthreads();
         float2 *o2 = (o == bufA) ? bufB : bufA;
         float2 *d = lds_fft<T, -1>(o, o2, n, P.tw, tid);
         const float fr = (float)P.f_R;
 #pragma unroll
-        for (int i = 0; i < MAXE; ++i)
+        for (int i = 0; i < E; ++i)
             if (i < E) {
                 const float2 idl = cscale(d[tid + T * i], invn);
                 Ie[i] = make_float2((1.0f - fr) * Ie[i].x + fr * idl.x, fr * idl.y);
@@ -250,30 +297,19 @@ __device__ void neval(const Inst1D &P, const float2 (&F)[MAXE], float2 (&out)[MA
         __syncthreads();
     }
     // plasma coefficients (zero unless the model is on and some rate is non-zero)
-    float c1[MAXE], c2[MAXE];
+    float c1[E], c2[E];
     bool pl = false;
 #pragma unroll
-    for (int i = 0; i < MAXE; ++i) {
+    for (int i = 0; i < E; ++i) {
         c1[i] = 0.0f;
         c2[i] = 0.0f;
     }
     if (P.use_plasma) {
-        bool any = false;
-#pragma unroll
-        for (int i = 0; i < MAXE; ++i)
-            if (i < E) {
-                const double I = (double)cabs2(a[i]) * P.inv_area;
-                double pw = 1.0;   // I^K by repeated multiplication (integer K)
-                for (int k = 0; k < P.K; ++k) pw *= I;
-                const float w = (float)(P.sigma_K * pw);
-                bufA[tid + T * i] = make_float2(w, (float)I);
-                any |= (w > 0.0f);
-            }
-        pl = __syncthreads_or(any ? 1 : 0) != 0;
+        pl = plasma_setup<T, E>(P, a, bufA, scan, tid);
         if (pl) {
             plasma_coeffs_1d<T>(bufA, n, P, scan, tid);
 #pragma unroll
-            for (int i = 0; i < MAXE; ++i)
+            for (int i = 0; i < E; ++i)
                 if (i < E) {
                     const float2 c = bufA[tid + T * i];
                     c1[i] = c.x;
@@ -287,12 +323,12 @@ __device__ void neval(const Inst1D &P, const float2 (&F)[MAXE], float2 (&out)[MA
     if (P.use_ss) {
         // out = ifft-free omega form: iγ uppe(ω) FFT(P_NL) + FFT(plasma)
 #pragma unroll
-        for (int i = 0; i < MAXE; ++i)
+        for (int i = 0; i < E; ++i)
             if (i < E) bufA[tid + T * i] = cmul(a[i], Ie[i]);
         __syncthreads();
         float2 *k = lds_fft<T, 1>(bufA, bufB, n, P.tw, tid);
 #pragma unroll
-        for (int i = 0; i < MAXE; ++i)
+        for (int i = 0; i < E; ++i)
             if (i < E) {
                 const int e = tid + T * i;
                 const float s = g * P.uppe[e];
@@ -302,21 +338,21 @@ __device__ void neval(const Inst1D &P, const float2 (&F)[MAXE], float2 (&out)[MA
         __syncthreads();
         if (pl) {
 #pragma unroll
-            for (int i = 0; i < MAXE; ++i)
+            for (int i = 0; i < E; ++i)
                 if (i < E)
                     bufA[tid + T * i] = make_float2(-c1[i] * a[i].x + c2[i] * a[i].y,
                                                     -c1[i] * a[i].y - c2[i] * a[i].x);
             __syncthreads();
             float2 *q = lds_fft<T, 1>(bufA, bufB, n, P.tw, tid);
 #pragma unroll
-            for (int i = 0; i < MAXE; ++i)
+            for (int i = 0; i < E; ++i)
                 if (i < E) out[i] = cadd(out[i], q[tid + T * i]);
             __syncthreads();
         }
     } else {
         // iγ P_NL + plasma in time, one FFT
 #pragma unroll
-        for (int i = 0; i < MAXE; ++i)
+        for (int i = 0; i < E; ++i)
             if (i < E) {
                 const float2 p = cmul(a[i], Ie[i]);
                 bufA[tid + T * i] = make_float2(-g * p.y - c1[i] * a[i].x + c2[i] * a[i].y,
@@ -325,7 +361,7 @@ __device__ void neval(const Inst1D &P, const float2 (&F)[MAXE], float2 (&out)[MA
         __syncthreads();
         float2 *k = lds_fft<T, 1>(bufA, bufB, n, P.tw, tid);
 #pragma unroll
-        for (int i = 0; i < MAXE; ++i)
+        for (int i = 0; i < E; ++i)
             if (i < E) out[i] = k[tid + T * i];
         __syncthreads();
     }
@@ -341,14 +377,14 @@ __device__ __forceinline__ float2 ipexp(const Inst1D &P, int e, double s) {
 
 // One attempt of size h from state A (global) with FSAL term a5 (global).  Produces A4 in
 // A4o, N(A4) in a5o, returns err (all threads).  snap: history slot for A4's time samples.
-template <int T, int NMAX>
-__device__ double attempt(const Inst1D &P, double h, float2 (&A4o)[MAXE], float2 (&a5o)[MAXE],
+template <int T, int NMAX, int E>
+__device__ double attempt(const Inst1D &P, double h, float2 (&A4o)[E], float2 (&a5o)[E],
                           float2 *bufA, float2 *bufB, double *scan, int tid, float2 *snap) {
-    const int E = P.nt / T;
+    
     const float hf = (float)h;
-    float2 Ef[MAXE], v[MAXE], S[MAXE], x[MAXE], y[MAXE];
+    float2 Ef[E], v[E], S[E], x[E], y[E];
 #pragma unroll
-    for (int i = 0; i < MAXE; ++i)
+    for (int i = 0; i < E; ++i)
         if (i < E) {
             const int e = tid + T * i;
             Ef[i] = ipexp(P, e, 0.5 * h);
@@ -357,31 +393,31 @@ __device__ double attempt(const Inst1D &P, double h, float2 (&A4o)[MAXE], float2
             S[i] = caxpy(v[i], hf / 6.0f, a1);
             x[i] = caxpy(v[i], 0.5f * hf, a1);
         }
-    neval<T, NMAX>(P, x, y, bufA, bufB, scan, tid, nullptr);   // a2
+    neval<T, NMAX, E>(P, x, y, bufA, bufB, scan, tid, nullptr);   // a2
 #pragma unroll
-    for (int i = 0; i < MAXE; ++i)
+    for (int i = 0; i < E; ++i)
         if (i < E) {
             S[i] = caxpy(S[i], hf / 3.0f, y[i]);
             x[i] = caxpy(v[i], 0.5f * hf, y[i]);
         }
-    neval<T, NMAX>(P, x, y, bufA, bufB, scan, tid, nullptr);   // a3
+    neval<T, NMAX, E>(P, x, y, bufA, bufB, scan, tid, nullptr);   // a3
 #pragma unroll
-    for (int i = 0; i < MAXE; ++i)
+    for (int i = 0; i < E; ++i)
         if (i < E) {
             S[i] = caxpy(S[i], hf / 3.0f, y[i]);
             x[i] = cmul(Ef[i], caxpy(v[i], hf, y[i]));
         }
-    neval<T, NMAX>(P, x, y, bufA, bufB, scan, tid, nullptr);   // a4
+    neval<T, NMAX, E>(P, x, y, bufA, bufB, scan, tid, nullptr);   // a4
 #pragma unroll
-    for (int i = 0; i < MAXE; ++i)
+    for (int i = 0; i < E; ++i)
         if (i < E) {
             A4o[i] = caxpy(cmul(Ef[i], S[i]), hf / 6.0f, y[i]);   // beta + h/6 a4
             x[i] = y[i];                                          // keep a4
         }
-    neval<T, NMAX>(P, A4o, a5o, bufA, bufB, scan, tid, snap);  // a5
+    neval<T, NMAX, E>(P, A4o, a5o, bufA, bufB, scan, tid, snap);  // a5
     double sd = 0.0, sa = 0.0;
 #pragma unroll
-    for (int i = 0; i < MAXE; ++i)
+    for (int i = 0; i < E; ++i)
         if (i < E) {
             const float2 d = csub(x[i], a5o[i]);
             sd += (double)d.x * d.x + (double)d.y * d.y;
@@ -395,13 +431,13 @@ __device__ double attempt(const Inst1D &P, double h, float2 (&A4o)[MAXE], float2
     return (h / 10.0) * sqrt(sd / (double)P.nt) / na;
 }
 
-template <int T, int NMAX>
+template <int T, int NMAX, int E>
 __device__ void run_instance(const Inst1D &P, float2 *bufA, float2 *bufB, double *scan,
                              double *bc, int tid) {
-    const int n = P.nt, E = n / T;
+    constexpr int n = T * E;
     // A0 -> history slot 0 and state FFT(A0); a5 = N(A)
 #pragma unroll
-    for (int i = 0; i < MAXE; ++i)
+    for (int i = 0; i < E; ++i)
         if (i < E) {
             const int e = tid + T * i;
             const float2 a0 = P.a0[e];
@@ -411,17 +447,17 @@ __device__ void run_instance(const Inst1D &P, float2 *bufA, float2 *bufB, double
     __syncthreads();
     {
         float2 *r = lds_fft<T, 1>(bufA, bufB, n, P.tw, tid);
-        float2 A[MAXE], N0[MAXE];
+        float2 A[E], N0[E];
 #pragma unroll
-        for (int i = 0; i < MAXE; ++i)
+        for (int i = 0; i < E; ++i)
             if (i < E) {
                 A[i] = r[tid + T * i];
                 P.A[tid + T * i] = A[i];
             }
         __syncthreads();
-        neval<T, NMAX>(P, A, N0, bufA, bufB, scan, tid, nullptr);
+        neval<T, NMAX, E>(P, A, N0, bufA, bufB, scan, tid, nullptr);
 #pragma unroll
-        for (int i = 0; i < MAXE; ++i)
+        for (int i = 0; i < E; ++i)
             if (i < E) P.a5[tid + T * i] = N0[i];
     }
     // controller state (thread 0 owns it; broadcast through bc[])
@@ -431,7 +467,7 @@ __device__ void run_instance(const Inst1D &P, float2 *bufA, float2 *bufB, double
     double zlast = 0.0;
     if (nh > 0 && tid == 0) P.zhist[0] = 0.0;
     double err = 0.0;
-    float2 A4[MAXE], a5[MAXE];
+    float2 A4[E], a5[E];
     while (P.replay_hs ? (acc < P.replay_M) : (z < P.L)) {
         if (P.max_attempts > 0 && attempts >= P.max_attempts) break;
         if (P.replay_hs) h = P.replay_hs[acc];
@@ -439,7 +475,7 @@ __device__ void run_instance(const Inst1D &P, float2 *bufA, float2 *bufB, double
         const int slot = nh < P.max_hist ? nh : -1;   // no overwrite once the buffer is full
         float2 *snap = slot >= 0 ? P.hist + (size_t)slot * n : nullptr;
         double h_used = h;
-        err = attempt<T, NMAX>(P, h, A4, a5, bufA, bufB, scan, tid, snap);
+        err = attempt<T, NMAX, E>(P, h, A4, a5, bufA, bufB, scan, tid, snap);
         ++attempts;
         if (!(err == err) || isinf(err)) {
             nonfinite = 1;
@@ -463,7 +499,7 @@ __device__ void run_instance(const Inst1D &P, float2 *bufA, float2 *bufB, double
             if (h <= P.min_step) {
                 // forced step at min_step (erk43ip_method.py:344-367)
                 h_used = h;
-                err = attempt<T, NMAX>(P, h, A4, a5, bufA, bufB, scan, tid, snap);
+                err = attempt<T, NMAX, E>(P, h, A4, a5, bufA, bufB, scan, tid, snap);
                 ++attempts;
                 ++forced;
                 if (!(err == err) || isinf(err)) {
@@ -484,7 +520,7 @@ __device__ void run_instance(const Inst1D &P, float2 *bufA, float2 *bufB, double
                     k / P.ckpt_every < P.ckpt_cap) {
                     float2 *cp = P.ckpt + (size_t)(k / P.ckpt_every) * n;
 #pragma unroll
-                    for (int i = 0; i < MAXE; ++i)
+                    for (int i = 0; i < E; ++i)
                         if (i < E) cp[tid + T * i] = P.A[tid + T * i];
                 }
             } else {
@@ -495,7 +531,7 @@ __device__ void run_instance(const Inst1D &P, float2 *bufA, float2 *bufB, double
             // history rule of erk43ip_method.py:327-329
             record = (nh < 1000) || (z - zlast > P.L / 500.0);
 #pragma unroll
-            for (int i = 0; i < MAXE; ++i)
+            for (int i = 0; i < E; ++i)
                 if (i < E) {
                     P.A[tid + T * i] = A4[i];
                     P.a5[tid + T * i] = a5[i];
@@ -515,7 +551,7 @@ __device__ void run_instance(const Inst1D &P, float2 *bufA, float2 *bufB, double
     }
     if (P.outw) {
 #pragma unroll
-        for (int i = 0; i < MAXE; ++i)
+        for (int i = 0; i < E; ++i)
             if (i < E) P.outw[tid + T * i] = P.A[tid + T * i];
     }
     if (tid == 0) {
@@ -536,6 +572,13 @@ __device__ void run_instance(const Inst1D &P, float2 *bufA, float2 *bufB, double
         *P.st = s;
     }
     __syncthreads();
+}
+
+// instance of nt = T E points (E, the values per thread, fixed at compile time)
+template <int T, int NMAX, int E>
+__device__ __forceinline__ void run_e(const Inst1D &P, float2 *bufA, float2 *bufB, double *scan,
+                                      double *bc, int tid) {
+    if constexpr (T * E <= NMAX) run_instance<T, NMAX, E>(P, bufA, bufB, scan, bc, tid);
 }
 
 __global__ void zero_counter(int *c) { *c = 0; }
@@ -565,7 +608,12 @@ __global__ __launch_bounds__(T) void uppe1d_kernel(const Inst1D *__restrict__ in
             }
             continue;
         }
-        run_instance<T, NMAX>(P, bufA, bufB, scan, bc, tid);
+        switch (P.nt / T) {
+            case 1: run_e<T, NMAX, 1>(P, bufA, bufB, scan, bc, tid); break;
+            case 2: run_e<T, NMAX, 2>(P, bufA, bufB, scan, bc, tid); break;
+            case 4: run_e<T, NMAX, 4>(P, bufA, bufB, scan, bc, tid); break;
+            default: run_e<T, NMAX, 8>(P, bufA, bufB, scan, bc, tid); break;
+        }
     }
 }
 
@@ -647,25 +695,17 @@ __device__ double block_affine_entry(double A, double B, double *scan, int tid) 
 
 // Adds the plasma-term contribution to ab (time, natural mapping) given a (time) and
 // Pb = dL/dP (time).  Affine regime only (zero avalanche rate, no clamp); otherwise flags.
-template <int T, int NMAX>
-__device__ void plasma_adjoint_1d(const Adj1D &J, const float2 (&a)[MAXE], const float2 (&Pb)[MAXE],
-                                  float2 (&ab)[MAXE], float2 *bufA, float2 *bufB, double *scan,
+template <int T, int NMAX, int E>
+__device__ void plasma_adjoint_1d(const Adj1D &J, const float2 (&a)[E], const float2 (&Pb)[E],
+                                  float2 (&ab)[E], float2 *bufA, float2 *bufB, double *scan,
                                   int tid) {
     const Inst1D &P = J.P;
-    const int n = P.nt, E = n / T;
-    bool any = false, risk = false;
+    constexpr int n = T * E;
+    if (!plasma_setup<T, E>(P, a, bufA, scan, tid)) return;
+    bool risk = false;
 #pragma unroll
-    for (int i = 0; i < MAXE; ++i)
-        if (i < E) {
-            const double I = (double)cabs2(a[i]) * P.inv_area;
-            double pw = 1.0;
-            for (int k = 0; k < P.K; ++k) pw *= I;
-            const float w = (float)(P.sigma_K * pw);
-            bufA[tid + T * i] = make_float2(w, (float)I);
-            any |= (w > 0.0f);
-            risk |= ((double)w * P.f_rec > P.e_rec);
-        }
-    if (__syncthreads_or(any ? 1 : 0) == 0) return;
+    for (int i = 0; i < E; ++i)
+        risk |= ((double)bufA[tid + T * i].x * P.f_rec > P.e_rec);
     if (P.avalanche || __syncthreads_or(risk ? 1 : 0)) {
         if (tid == 0) atomicOr(J.flags, 1);
         return;
@@ -695,7 +735,7 @@ __device__ void plasma_adjoint_1d(const Adj1D &J, const float2 (&a)[MAXE], const
     double *dd = J.pscr, *wb = J.pscr + n, *Ib = J.pscr + 2 * n;
     // direct terms at every sample (natural mapping)
 #pragma unroll
-    for (int i = 0; i < MAXE; ++i)
+    for (int i = 0; i < E; ++i)
         if (i < E) {
             const int t = tid + T * i;
             const double rho = rho_s[t];
@@ -733,7 +773,7 @@ __device__ void plasma_adjoint_1d(const Adj1D &J, const float2 (&a)[MAXE], const
     }
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < MAXE; ++i)
+    for (int i = 0; i < E; ++i)
         if (i < E) {
             const double s = 2.0 * Ib[tid + T * i] * P.inv_area;
             ab[i] = cadd(ab[i], make_float2((float)(s * a[i].x), (float)(s * a[i].y)));
@@ -742,34 +782,34 @@ __device__ void plasma_adjoint_1d(const Adj1D &J, const float2 (&a)[MAXE], const
 }
 
 // Fb = N^*(F; yb): adjoint of y = N(F), accumulating dL/dU (per omega) and dL/dgamma.
-template <int T, int NMAX>
-__device__ void adj_neval(const Adj1D &J, const float2 (&F)[MAXE], const float2 (&yb)[MAXE],
-                          float2 (&Fb)[MAXE], double &gG, float2 *bufA, float2 *bufB,
+template <int T, int NMAX, int E>
+__device__ void adj_neval(const Adj1D &J, const float2 (&F)[E], const float2 (&yb)[E],
+                          float2 (&Fb)[E], double &gG, float2 *bufA, float2 *bufB,
                           double *scan, int tid) {
     const Inst1D &P = J.P;
-    const int n = P.nt, E = n / T;
+    constexpr int n = T * E;
     const float invn = 1.0f / (float)n;
     const float g = (float)P.gamma;
 #pragma unroll
-    for (int i = 0; i < MAXE; ++i)
+    for (int i = 0; i < E; ++i)
         if (i < E) bufA[tid + T * i] = F[i];
     __syncthreads();
     float2 *r = lds_fft<T, -1>(bufA, bufB, n, P.tw, tid);
-    float2 a[MAXE], pb[MAXE], Pb[MAXE], ab[MAXE];
+    float2 a[E], pb[E], Pb[E], ab[E];
 #pragma unroll
-    for (int i = 0; i < MAXE; ++i)
+    for (int i = 0; i < E; ++i)
         if (i < E) a[i] = cscale(r[tid + T * i], invn);
     __syncthreads();
     double gsum = 0.0;
     if (P.use_ss) {
 #pragma unroll
-        for (int i = 0; i < MAXE; ++i)
+        for (int i = 0; i < E; ++i)
             if (i < E) bufA[tid + T * i] = cscale(a[i], cabs2(a[i]));
         __syncthreads();
         float2 *q = lds_fft<T, 1>(bufA, bufB, n, P.tw, tid);
-        float2 Qb[MAXE];
+        float2 Qb[E];
 #pragma unroll
-        for (int i = 0; i < MAXE; ++i)
+        for (int i = 0; i < E; ++i)
             if (i < E) {
                 const int e = tid + T * i;
                 const float2 Q = q[e];
@@ -781,33 +821,33 @@ __device__ void adj_neval(const Adj1D &J, const float2 (&F)[MAXE], const float2 
             }
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < MAXE; ++i)
+        for (int i = 0; i < E; ++i)
             if (i < E) bufA[tid + T * i] = Qb[i];
         __syncthreads();
         float2 *r2 = lds_fft<T, -1>(bufA, bufB, n, P.tw, tid);   // n IFFT
 #pragma unroll
-        for (int i = 0; i < MAXE; ++i)
+        for (int i = 0; i < E; ++i)
             if (i < E) pb[i] = r2[tid + T * i];
         __syncthreads();
         if (P.use_plasma) {
 #pragma unroll
-            for (int i = 0; i < MAXE; ++i)
+            for (int i = 0; i < E; ++i)
                 if (i < E) bufA[tid + T * i] = yb[i];
             __syncthreads();
             float2 *r3 = lds_fft<T, -1>(bufA, bufB, n, P.tw, tid);
 #pragma unroll
-            for (int i = 0; i < MAXE; ++i)
+            for (int i = 0; i < E; ++i)
                 if (i < E) Pb[i] = r3[tid + T * i];
             __syncthreads();
         }
     } else {
 #pragma unroll
-        for (int i = 0; i < MAXE; ++i)
+        for (int i = 0; i < E; ++i)
             if (i < E) bufA[tid + T * i] = yb[i];
         __syncthreads();
         float2 *r2 = lds_fft<T, -1>(bufA, bufB, n, P.tw, tid);   // wb = n IFFT(yb)
 #pragma unroll
-        for (int i = 0; i < MAXE; ++i)
+        for (int i = 0; i < E; ++i)
             if (i < E) {
                 const float2 w = r2[tid + T * i];
                 const float2 p = cscale(a[i], cabs2(a[i]));
@@ -819,21 +859,21 @@ __device__ void adj_neval(const Adj1D &J, const float2 (&F)[MAXE], const float2 
     }
     // p = |a|^2 a:  ab = 2 |a|^2 pb + conj(pb) a^2
 #pragma unroll
-    for (int i = 0; i < MAXE; ++i)
+    for (int i = 0; i < E; ++i)
         if (i < E) {
             const float m = 2.0f * cabs2(a[i]);
             const float2 a2 = cmul(a[i], a[i]);
             const float2 cp = make_float2(pb[i].x, -pb[i].y);
             ab[i] = cadd(cscale(pb[i], m), cmul(cp, a2));
         }
-    if (P.use_plasma) plasma_adjoint_1d<T, NMAX>(J, a, Pb, ab, bufA, bufB, scan, tid);
+    if (P.use_plasma) plasma_adjoint_1d<T, NMAX, E>(J, a, Pb, ab, bufA, bufB, scan, tid);
 #pragma unroll
-    for (int i = 0; i < MAXE; ++i)
+    for (int i = 0; i < E; ++i)
         if (i < E) bufA[tid + T * i] = ab[i];
     __syncthreads();
     float2 *q2 = lds_fft<T, 1>(bufA, bufB, n, P.tw, tid);
 #pragma unroll
-    for (int i = 0; i < MAXE; ++i)
+    for (int i = 0; i < E; ++i)
         if (i < E) Fb[i] = cscale(q2[tid + T * i], invn);
     __syncthreads();
     gG += block_sum<T>(gsum, scan + 32, tid);
@@ -845,14 +885,14 @@ __device__ __forceinline__ void acc_gD(double *gD, int e, double s, float2 yb, f
     gD[2 * e + 1] += s * ((double)yb.y * y.x - (double)yb.x * y.y);
 }
 
-template <int T, int NMAX>
+template <int T, int NMAX, int E>
 __device__ void run_adjoint(const Adj1D &J, float2 *bufA, float2 *bufB, double *scan, int tid) {
     const Inst1D &P = J.P;
-    const int n = P.nt, E = n / T;
+    constexpr int n = T * E;
     const size_t nn = (size_t)n;
-    float2 lam[MAXE];
+    float2 lam[E];
 #pragma unroll
-    for (int i = 0; i < MAXE; ++i)
+    for (int i = 0; i < E; ++i)
         if (i < E) {
             const int e = tid + T * i;
             lam[i] = J.lam[e];
@@ -867,18 +907,18 @@ __device__ void run_adjoint(const Adj1D &J, float2 *bufA, float2 *bufB, double *
         const int j0 = sg * J.C;
         const int j1 = (j0 + J.C < J.M) ? j0 + J.C : J.M;
         // recompute the segment forward from its checkpoint, storing A, N(A), a2, a3, a4
-        float2 A[MAXE], k1[MAXE];
+        float2 A[E], k1[E];
 #pragma unroll
-        for (int i = 0; i < MAXE; ++i)
+        for (int i = 0; i < E; ++i)
             if (i < E) A[i] = J.ckpt[(size_t)sg * nn + tid + T * i];
-        neval<T, NMAX>(P, A, k1, bufA, bufB, scan, tid, nullptr);
+        neval<T, NMAX, E>(P, A, k1, bufA, bufB, scan, tid, nullptr);
         for (int j = j0; j < j1; ++j) {
             float2 *s5 = J.seg + (size_t)(j - j0) * 5 * nn;
             const double h = J.hs[j];
             const float hf = (float)h;
-            float2 Ef[MAXE], v[MAXE], S[MAXE], x[MAXE], y[MAXE];
+            float2 Ef[E], v[E], S[E], x[E], y[E];
 #pragma unroll
-            for (int i = 0; i < MAXE; ++i)
+            for (int i = 0; i < E; ++i)
                 if (i < E) {
                     const int e = tid + T * i;
                     s5[e] = A[i];
@@ -889,30 +929,30 @@ __device__ void run_adjoint(const Adj1D &J, float2 *bufA, float2 *bufB, double *
                     S[i] = caxpy(v[i], hf / 6.0f, a1);
                     x[i] = caxpy(v[i], 0.5f * hf, a1);
                 }
-            neval<T, NMAX>(P, x, y, bufA, bufB, scan, tid, nullptr);   // a2
+            neval<T, NMAX, E>(P, x, y, bufA, bufB, scan, tid, nullptr);   // a2
 #pragma unroll
-            for (int i = 0; i < MAXE; ++i)
+            for (int i = 0; i < E; ++i)
                 if (i < E) {
                     s5[2 * nn + tid + T * i] = y[i];
                     S[i] = caxpy(S[i], hf / 3.0f, y[i]);
                     x[i] = caxpy(v[i], 0.5f * hf, y[i]);
                 }
-            neval<T, NMAX>(P, x, y, bufA, bufB, scan, tid, nullptr);   // a3
+            neval<T, NMAX, E>(P, x, y, bufA, bufB, scan, tid, nullptr);   // a3
 #pragma unroll
-            for (int i = 0; i < MAXE; ++i)
+            for (int i = 0; i < E; ++i)
                 if (i < E) {
                     s5[3 * nn + tid + T * i] = y[i];
                     S[i] = caxpy(S[i], hf / 3.0f, y[i]);
                     x[i] = cmul(Ef[i], caxpy(v[i], hf, y[i]));
                 }
-            neval<T, NMAX>(P, x, y, bufA, bufB, scan, tid, nullptr);   // a4
+            neval<T, NMAX, E>(P, x, y, bufA, bufB, scan, tid, nullptr);   // a4
 #pragma unroll
-            for (int i = 0; i < MAXE; ++i)
+            for (int i = 0; i < E; ++i)
                 if (i < E) {
                     s5[4 * nn + tid + T * i] = y[i];
                     A[i] = caxpy(cmul(Ef[i], S[i]), hf / 6.0f, y[i]);
                 }
-            if (j + 1 < j1) neval<T, NMAX>(P, A, k1, bufA, bufB, scan, tid, nullptr);
+            if (j + 1 < j1) neval<T, NMAX, E>(P, A, k1, bufA, bufB, scan, tid, nullptr);
             __syncthreads();
         }
         // backward through the segment
@@ -921,10 +961,10 @@ __device__ void run_adjoint(const Adj1D &J, float2 *bufA, float2 *bufB, double *
             const double h = J.hs[j];
             const float hf = (float)h;
             const double sh = 0.5 * h;
-            float2 Ef[MAXE], t1[MAXE], t2[MAXE];
+            float2 Ef[E], t1[E], t2[E];
             // stage A4 = E S + h/6 a4
 #pragma unroll
-            for (int i = 0; i < MAXE; ++i)
+            for (int i = 0; i < E; ++i)
                 if (i < E) {
                     const int e = tid + T * i;
                     Ef[i] = ipexp(P, e, sh);
@@ -943,10 +983,10 @@ __device__ void run_adjoint(const Adj1D &J, float2 *bufA, float2 *bufB, double *
                     t1[i] = cmul(Ef[i], caxpy(v, hf, a3));   // x4
                     t2[i] = cscale(lam[i], hf / 6.0f);        // a4b
                 }
-            float2 xb[MAXE];
-            adj_neval<T, NMAX>(J, t1, t2, xb, gG, bufA, bufB, scan, tid);   // x4b
+            float2 xb[E];
+            adj_neval<T, NMAX, E>(J, t1, t2, xb, gG, bufA, bufB, scan, tid);   // x4b
 #pragma unroll
-            for (int i = 0; i < MAXE; ++i)
+            for (int i = 0; i < E; ++i)
                 if (i < E) {
                     const int e = tid + T * i;
                     acc_gD(J.gD, e, sh, xb[i], t1[i]);
@@ -957,9 +997,9 @@ __device__ void run_adjoint(const Adj1D &J, float2 *bufA, float2 *bufB, double *
                     t1[i] = caxpy(v, 0.5f * hf, s5[2 * nn + e]);   // x3
                     t2[i] = a3b[e];
                 }
-            adj_neval<T, NMAX>(J, t1, t2, xb, gG, bufA, bufB, scan, tid);   // x3b
+            adj_neval<T, NMAX, E>(J, t1, t2, xb, gG, bufA, bufB, scan, tid);   // x3b
 #pragma unroll
-            for (int i = 0; i < MAXE; ++i)
+            for (int i = 0; i < E; ++i)
                 if (i < E) {
                     const int e = tid + T * i;
                     vb[e] = cadd(vb[e], xb[i]);
@@ -968,9 +1008,9 @@ __device__ void run_adjoint(const Adj1D &J, float2 *bufA, float2 *bufB, double *
                     t1[i] = caxpy(v, 0.5f * hf, cmul(Ef[i], s5[nn + e]));   // x2
                     t2[i] = a2b[e];
                 }
-            adj_neval<T, NMAX>(J, t1, t2, xb, gG, bufA, bufB, scan, tid);   // x2b
+            adj_neval<T, NMAX, E>(J, t1, t2, xb, gG, bufA, bufB, scan, tid);   // x2b
 #pragma unroll
-            for (int i = 0; i < MAXE; ++i)
+            for (int i = 0; i < E; ++i)
                 if (i < E) {
                     const int e = tid + T * i;
                     vb[e] = cadd(vb[e], xb[i]);
@@ -980,9 +1020,9 @@ __device__ void run_adjoint(const Adj1D &J, float2 *bufA, float2 *bufB, double *
                     t1[i] = s5[e];                      // A
                     t2[i] = cmulc(a1bt, Ef[i]);         // k1b
                 }
-            adj_neval<T, NMAX>(J, t1, t2, xb, gG, bufA, bufB, scan, tid);   // from k1 = N(A)
+            adj_neval<T, NMAX, E>(J, t1, t2, xb, gG, bufA, bufB, scan, tid);   // from k1 = N(A)
 #pragma unroll
-            for (int i = 0; i < MAXE; ++i)
+            for (int i = 0; i < E; ++i)
                 if (i < E) {
                     const int e = tid + T * i;
                     const float2 v = cmul(Ef[i], s5[e]);
@@ -993,9 +1033,15 @@ __device__ void run_adjoint(const Adj1D &J, float2 *bufA, float2 *bufB, double *
         }
     }
 #pragma unroll
-    for (int i = 0; i < MAXE; ++i)
+    for (int i = 0; i < E; ++i)
         if (i < E) J.lam[tid + T * i] = lam[i];
     if (tid == 0) *J.gG = gG;
+}
+
+template <int T, int NMAX, int E>
+__device__ __forceinline__ void adjoint_e(const Adj1D &J, float2 *bufA, float2 *bufB,
+                                          double *scan, int tid) {
+    if constexpr (T * E <= NMAX) run_adjoint<T, NMAX, E>(J, bufA, bufB, scan, tid);
 }
 
 template <int T, int NMAX>
@@ -1009,7 +1055,12 @@ __global__ __launch_bounds__(T) void uppe1d_adjoint_kernel(const Adj1D *__restri
             if (threadIdx.x == 0) atomicOr(J.flags, 2);
             continue;
         }
-        run_adjoint<T, NMAX>(J, bufA, bufB, scan, threadIdx.x);
+        switch (J.P.nt / T) {
+            case 1: adjoint_e<T, NMAX, 1>(J, bufA, bufB, scan, threadIdx.x); break;
+            case 2: adjoint_e<T, NMAX, 2>(J, bufA, bufB, scan, threadIdx.x); break;
+            case 4: adjoint_e<T, NMAX, 4>(J, bufA, bufB, scan, threadIdx.x); break;
+            default: adjoint_e<T, NMAX, 8>(J, bufA, bufB, scan, threadIdx.x); break;
+        }
         __syncthreads();
     }
 }
