@@ -108,6 +108,28 @@ int uppe3d_state_ptr(uppe3d_plan *p, void *stream, void **state);
 int uppe3d_get_state(uppe3d_plan *p, void *dst, void *stream);
 int uppe3d_set_state(uppe3d_plan *p, const void *src, void *stream);
 
+/* Diagnostics of the 3+1D MPC runner (src/3+1D-ERK4(3)IP UPPE/sim.py:140-220) as device
+ * reductions over the current state, in normalised units (A / a_scale; the caller applies
+ * a_scale^2, dx, dy, dt).  out: float64 device buffer of 8 nt + nx ny + 5 values:
+ *   [0, nt)        sum_xy |B(x,y,w)|^2                 (B = FFT_t(A / a_scale), FFT order)
+ *   [nt, 3 nt)     re, im of sum_xy B(x,y,w) m(x,y)     m = exp(-(x^2+y^2)/w0_mode^2)
+ *   [3 nt, 4 nt)   sum_xy |A(x,y,t)|^2                 (t in natural order)
+ *   [4 nt, 6 nt)   re, im of sum_xy A(x,y,t) m(x,y)
+ *   [6 nt, 7 nt)   |B(ix0,iy0,w)|^2;  [7 nt, 8 nt) |A(ix0,iy0,t)|^2
+ *   [8 nt, 8 nt + nx ny)   fluence sum_t |A(x,y,t)|^2, x-major
+ *   then sum f, x mean, y mean, x rms, y rms of the fluence (sim.py:145-157).
+ * x = x0 + i dx, y = y0 + j dy.  work: uppe3d_diag_workspace_size bytes (device).
+ * uppe3d_field_at_time: out_xy complex64 (nx, ny) = A(x, y, t_it) / a_scale.
+ * uppe3d_apply_mirror: the curved mirror of sim.py:82-96 and a loss factor on the state,
+ * B *= amp exp(-i (omega0 + omega) (x^2 + y^2) / (c R)) (R = 0: amp only).
+ * No counterpart for the ABI itself: the reference computes these with CuPy array ops. */
+int uppe3d_diag_workspace_size(const uppe3d_desc *d, size_t *bytes);
+int uppe3d_diagnostics(uppe3d_plan *p, double w0_mode, double x0, double dx, double y0,
+                       double dy, int ix0, int iy0, void *work, void *out, void *stream);
+int uppe3d_field_at_time(uppe3d_plan *p, int it, void *out_xy, void *stream);
+int uppe3d_apply_mirror(uppe3d_plan *p, double R, double amp, double x0, double dx, double y0,
+                        double dy, void *stream);
+
 /* Adjoint of the 3+1D step (reverse mode through a frozen step sequence; no counterpart in
  * the reference).  uppe3d_set_stage_record makes every following attempt copy its stage
  * fields X2, X3, X4 (the inputs of the 2nd..4th nonlinear evaluations, same layout as the
@@ -177,6 +199,28 @@ int uppe1d_propagate(const uppe1d_desc *d, const double *D_re, const double *D_i
                      const double *uppe_factor, const double *H_re, const double *H_im,
                      const void *a0, void *workspace, void *hist, void *zhist, void *out_w,
                      const uppe1d_record *rec, void *stream, uppe1d_status *st);
+
+/* Slice chain (cavity / multipass drivers; replaces the host loop of gas.py:113-147 and
+ * "gas copy.py":115-141 that calls propagate once per slice with a new gamma).  All slices run
+ * in order in one kernel launch, with no host round trip between them.  Slice i is a full
+ * adaptive propagation (z = 0, h = max_step/100, fresh controller and history) over L with
+ * its own gamma, tol and step bounds, whose input is the field the reference loop hands on:
+ * A_hist[-1] of slice i-1, its last recorded snapshot (the final field unless the history
+ * rule skipped the last step).  d: the pulse and model (its L, tol, step bounds and gamma are
+ * not used); slices: host array of nslices.  hist/zhist (optional, max_hist snapshots in
+ * total): every slice's history, slice i's st[i].nhist snapshots after those of slices
+ * 0..i-1 (st[i].overflow bit 0 when the buffer ran out).  out_t (optional): complex64
+ * (nslices, nt), time domain, the field slice i hands on.  out_w (optional): complex64 (nt),
+ * final A(omega) of the last slice.  st: host array of nslices.  Synchronises the stream. */
+typedef struct {
+    double L, gamma, tol, max_step, min_step;
+} uppe1d_slice;
+int uppe1d_slices_workspace_size(const uppe1d_desc *d, int nslices, size_t *bytes);
+int uppe1d_propagate_slices(const uppe1d_desc *d, int nslices, const uppe1d_slice *slices,
+                            const double *D_re, const double *D_im, const double *uppe_factor,
+                            const double *H_re, const double *H_im, const void *a0,
+                            void *workspace, void *hist, void *zhist, void *out_t, void *out_w,
+                            void *stream, uppe1d_status *st);
 
 /* Batched sweep: n instances of uneven nt, each with its own adaptive step, run by one
  * persistent kernel (workgroups take instances from a device counter).  Arrays are

@@ -51,6 +51,11 @@ class Uppe1DRecord(ctypes.Structure):
                 ("replay_M", c_int)]
 
 
+class Uppe1DSlice(ctypes.Structure):
+    _fields_ = [("L", c_double), ("gamma", c_double), ("tol", c_double),
+                ("max_step", c_double), ("min_step", c_double)]
+
+
 # name -> (restype, argtypes); must match include/uppe.h
 SIGNATURES = {
     "uppe_last_error": (ctypes.c_char_p, []),
@@ -71,6 +76,12 @@ SIGNATURES = {
     "uppe3d_state_ptr": (c_int, [c_void_p, c_void_p, P(c_void_p)]),
     "uppe3d_get_state": (c_int, [c_void_p, c_void_p, c_void_p]),
     "uppe3d_set_state": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "uppe3d_diag_workspace_size": (c_int, [P(Uppe3DDesc), P(c_size_t)]),
+    "uppe3d_diagnostics": (c_int, [c_void_p, c_double, c_double, c_double, c_double, c_double,
+                                   c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "uppe3d_field_at_time": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
+    "uppe3d_apply_mirror": (c_int, [c_void_p, c_double, c_double, c_double, c_double, c_double,
+                                    c_double, c_void_p]),
     "uppe3d_set_stage_record": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "uppe3d_adjoint_workspace_size": (c_int, [P(Uppe3DDesc), P(c_size_t)]),
     "uppe3d_adjoint_step": (c_int, [c_void_p, c_double] + [c_void_p] * 7),
@@ -78,6 +89,9 @@ SIGNATURES = {
     "uppe1d_workspace_size": (c_int, [P(Uppe1DDesc), P(c_size_t)]),
     "uppe1d_propagate": (c_int, [P(Uppe1DDesc)] + [c_void_p] * 5
                          + [c_void_p] * 5 + [P(Uppe1DRecord), c_void_p, P(Uppe1DStatus)]),
+    "uppe1d_slices_workspace_size": (c_int, [P(Uppe1DDesc), c_int, P(c_size_t)]),
+    "uppe1d_propagate_slices": (c_int, [P(Uppe1DDesc), c_int, P(Uppe1DSlice)] + [c_void_p] * 5
+                                + [c_void_p] * 6 + [c_void_p, P(Uppe1DStatus)]),
     "uppe1d_batch_workspace_size": (c_int, [c_int, P(Uppe1DDesc), P(c_size_t)]),
     "uppe1d_batch_prepare": (c_int, [c_int, P(Uppe1DDesc)] + [c_void_p] * 5
                              + [c_void_p] * 5 + [P(Uppe1DRecord), c_void_p]),

@@ -618,6 +618,199 @@ __global__ __launch_bounds__(T) void uppe1d_kernel(const Inst1D *__restrict__ in
 }
 
 // ---------------------------------------------------------------------------------------
+// Slice chain (cavity / multipass drivers, gas.py:113-147): slices run in order by one
+// workgroup.  Each slice is a fresh adaptive propagation (z = 0, h = max_step/100, fresh
+// controller, history slot 0 = its input) with its own length, gamma, tol and step bounds,
+// whose input is what the reference loop hands on: A_hist[-1] of the previous slice, i.e. the
+// state at its last recorded snapshot (kept in lastrec, omega domain).
+struct Slice1D {
+    double L, gamma, tol, max_step, min_step;
+};
+
+struct Chain1D {
+    Inst1D P;              // the pulse and model; P.L, tol, step bounds and gamma are not used
+    const Slice1D *slices;
+    Stat1D *slice_st;      // [nslices]
+    float2 *lastrec;       // omega state at the last recorded snapshot [nt]
+    float2 *out_t;         // [nslices][nt] time domain (the field each slice hands on), or null
+    int nslices, pad_;
+};
+
+template <int T, int NMAX, int E>
+__device__ void run_chain(const Chain1D &C, float2 *bufA, float2 *bufB, double *scan, int tid) {
+    constexpr int n = T * E;
+    const Inst1D &Q = C.P;
+    int used = 0;   // history snapshots stored by the previous slices
+    for (int sl = 0; sl < C.nslices; ++sl) {
+        Inst1D P = Q;
+        const Slice1D S = C.slices[sl];
+        P.L = S.L;
+        P.gamma = S.gamma;
+        P.tol = S.tol;
+        P.max_step = S.max_step;
+        P.min_step = S.min_step;
+        const int cap = Q.max_hist > used ? Q.max_hist - used : 0;
+        float2 *hist = cap > 0 ? Q.hist + (size_t)used * n : nullptr;
+        double *zhist = cap > 0 ? Q.zhist + used : nullptr;
+        // input: time-domain a0 (slice 0) or the previous slice's last recorded state
+        float2 A[E], N0[E];
+        if (sl == 0) {
+#pragma unroll
+            for (int i = 0; i < E; ++i)
+                if (i < E) bufA[tid + T * i] = Q.a0[tid + T * i];
+            __syncthreads();
+            float2 *r = lds_fft<T, 1>(bufA, bufB, n, P.tw, tid);
+#pragma unroll
+            for (int i = 0; i < E; ++i)
+                if (i < E) A[i] = r[tid + T * i];
+            __syncthreads();
+        } else {
+#pragma unroll
+            for (int i = 0; i < E; ++i)
+                if (i < E) A[i] = C.lastrec[tid + T * i];
+        }
+#pragma unroll
+        for (int i = 0; i < E; ++i)
+            if (i < E) {
+                P.A[tid + T * i] = A[i];
+                C.lastrec[tid + T * i] = A[i];
+            }
+        __syncthreads();
+        // N(A); its inverse transform also writes the input's time samples to slot 0
+        neval<T, NMAX, E>(P, A, N0, bufA, bufB, scan, tid, hist);
+#pragma unroll
+        for (int i = 0; i < E; ++i)
+            if (i < E) P.a5[tid + T * i] = N0[i];
+        double z = 0.0, h = P.max_step / 100.0, eprev = -1.0, err = 0.0, zrec = 0.0;
+        int nrec = 1, nh = cap > 0 ? 1 : 0, attempts = 0, acc = 0, rej = 0, forced = 0;
+        int nonfinite = 0, overflow = (Q.max_hist > 0 && cap == 0) ? 1 : 0;
+        if (nh > 0 && tid == 0) zhist[0] = 0.0;
+        float2 A4[E], a5[E];
+        while (z < P.L) {
+            if (P.max_attempts > 0 && attempts >= P.max_attempts) break;
+            if (z + h > P.L) h = P.L - z;
+            const int slot = nh < cap ? nh : -1;
+            float2 *snap = slot >= 0 ? hist + (size_t)slot * n : nullptr;
+            err = attempt<T, NMAX, E>(P, h, A4, a5, bufA, bufB, scan, tid, snap);
+            ++attempts;
+            if (!(err == err) || isinf(err)) {
+                nonfinite = 1;
+                break;
+            }
+            bool accept = err < P.tol;
+            if (accept) {
+                z += h;
+                double fac = 0.95 * pow(P.tol / (err + 1e-30), 0.25);
+                if (eprev >= 0.0 && eprev > 1e-30) fac *= pow(eprev / (err + 1e-30), 0.1);
+                eprev = err;
+                const double hn = h * (fac < 2.0 ? fac : 2.0);
+                h = hn < P.max_step ? hn : P.max_step;
+                ++acc;
+            } else {
+                ++rej;
+                const double fac = 0.95 * pow(P.tol / (err + 1e-30), 0.25);
+                const double hn = h * (fac > 0.1 ? fac : 0.1);
+                h = hn > P.min_step ? hn : P.min_step;
+                if (h <= P.min_step) {
+                    // forced step at min_step (erk43ip_method.py:344-367)
+                    err = attempt<T, NMAX, E>(P, h, A4, a5, bufA, bufB, scan, tid, snap);
+                    ++attempts;
+                    ++forced;
+                    if (!(err == err) || isinf(err)) {
+                        nonfinite = 1;
+                        break;
+                    }
+                    z += h;
+                    eprev = -1.0;
+                    accept = true;
+                }
+            }
+            if (accept) {
+                // history rule of erk43ip_method.py:327-329, counted independently of storage
+                const bool record = (nrec < 1000) || (z - zrec > P.L / 500.0);
+#pragma unroll
+                for (int i = 0; i < E; ++i)
+                    if (i < E) {
+                        P.A[tid + T * i] = A4[i];
+                        P.a5[tid + T * i] = a5[i];
+                        if (record) C.lastrec[tid + T * i] = A4[i];
+                    }
+                if (record) {
+                    ++nrec;
+                    zrec = z;
+                    if (cap > 0) {
+                        if (slot >= 0) {
+                            if (tid == 0) zhist[slot] = z;
+                            ++nh;
+                        } else {
+                            overflow = 1;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        if (tid == 0) {
+            Stat1D s;
+            s.z = z;
+            s.h = h;
+            s.err = err;
+            s.attempts = attempts;
+            s.accepted = acc;
+            s.rejected = rej;
+            s.forced = forced;
+            s.nhist = cap > 0 ? nh : 0;
+            s.done = z >= P.L ? 1 : 0;
+            s.nonfinite = nonfinite;
+            s.fsal_hits = attempts;
+            s.hist_overflow = overflow;
+            s.rec_overflow = 0;
+            C.slice_st[sl] = s;
+        }
+        used += cap > 0 ? nh : 0;
+        if (C.out_t) {
+            // the field this slice hands on, in the time domain
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < E; ++i)
+                if (i < E) bufA[tid + T * i] = C.lastrec[tid + T * i];
+            __syncthreads();
+            float2 *r = lds_fft<T, -1>(bufA, bufB, n, P.tw, tid);
+            const float invn = 1.0f / (float)n;
+#pragma unroll
+            for (int i = 0; i < E; ++i)
+                if (i < E) C.out_t[(size_t)sl * n + tid + T * i] = cscale(r[tid + T * i], invn);
+        }
+        __syncthreads();
+        if (__syncthreads_or(nonfinite)) break;
+    }
+    if (Q.outw) {
+#pragma unroll
+        for (int i = 0; i < E; ++i)
+            if (i < E) Q.outw[tid + T * i] = Q.A[tid + T * i];
+    }
+    __syncthreads();
+}
+
+template <int T, int NMAX>
+__global__ __launch_bounds__(T) void uppe1d_chain_kernel(const Chain1D *__restrict__ chain) {
+    __shared__ float2 bufA[NMAX], bufB[NMAX];
+    __shared__ double scan[64];
+    const Chain1D C = *chain;
+    const int nt = C.P.nt, tid = threadIdx.x;
+    if (nt > NMAX || nt < T || nt % T != 0 || nt / T > MAXE) {
+        if (tid == 0) C.slice_st[0].attempts = -1;   // configuration rejected on the device
+        return;
+    }
+    switch (nt / T) {
+        case 1: if constexpr (T <= NMAX) run_chain<T, NMAX, 1>(C, bufA, bufB, scan, tid); break;
+        case 2: if constexpr (2 * T <= NMAX) run_chain<T, NMAX, 2>(C, bufA, bufB, scan, tid); break;
+        case 4: if constexpr (4 * T <= NMAX) run_chain<T, NMAX, 4>(C, bufA, bufB, scan, tid); break;
+        default: if constexpr (8 * T <= NMAX) run_chain<T, NMAX, 8>(C, bufA, bufB, scan, tid); break;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
 // Adjoint (reverse mode) of the frozen accepted-step sequence.  Gradient convention for a
 // complex quantity z: zb = dL/dRe z + i dL/dIm z (so y = c z gives zb = conj(c) yb and
 // y = M z gives zb = M^H yb; FFT^H = n IFFT, IFFT^H = FFT / n).
@@ -1286,6 +1479,90 @@ int status(void *workspace, int n, hipStream_t s, uppe1d_status *st) {
 
 int default_threads(int nt) { return nt <= 2048 ? 256 : nt / MAXE; }
 
+static_assert(sizeof(Slice1D) == sizeof(uppe1d_slice), "slice layout");
+
+size_t chain_bytes(int nt, int nslices) {
+    return align256(sizeof(Chain1D)) + align256(sizeof(Slice1D) * (size_t)nslices) +
+           align256(sizeof(Stat1D) * (size_t)nslices) + align256(8 * (size_t)nt);
+}
+
+int propagate_slices(const uppe1d_desc *d, int nslices, const uppe1d_slice *slices,
+                     const double *D_re, const double *D_im, const double *uppe_factor,
+                     const double *H_re, const double *H_im, const void *a0, void *workspace,
+                     void *hist, void *zhist, void *out_t, void *out_w, hipStream_t s,
+                     uppe1d_status *st) {
+    UPPE_CHECK(d && slices && st && nslices > 0, "uppe1d_propagate_slices: null argument");
+    for (int i = 0; i < nslices; ++i) {
+        const uppe1d_slice &c = slices[i];
+        UPPE_CHECK(c.tol > 0.0 && c.max_step > 0.0 && c.min_step >= 0.0 && c.L >= 0.0,
+                   "uppe1d_propagate_slices: slice %d has bad tolerance / step arguments", i);
+    }
+    // the instance (tables, state, history pointers) as for one propagation
+    int r = prepare(1, d, D_re, D_im, uppe_factor, H_re, H_im, a0, workspace, hist, zhist,
+                    out_w, nullptr, s);
+    if (r) return r;
+    char *ws = (char *)workspace;
+    Chain1D C;
+    memset(&C, 0, sizeof C);
+    UPPE_HIP(hipMemcpy(&C.P, ws + align256(sizeof(Header1D)), sizeof(Inst1D),
+                       hipMemcpyDeviceToHost));
+    char *cb = ws + header_bytes(1) + inst_bytes(d->nt);
+    size_t o = align256(sizeof(Chain1D));
+    C.slices = (const Slice1D *)(cb + o);
+    o += align256(sizeof(Slice1D) * (size_t)nslices);
+    C.slice_st = (Stat1D *)(cb + o);
+    o += align256(sizeof(Stat1D) * (size_t)nslices);
+    C.lastrec = (float2 *)(cb + o);
+    C.out_t = (float2 *)out_t;
+    C.nslices = nslices;
+    std::vector<Stat1D> zero(nslices);
+    memset(zero.data(), 0, sizeof(Stat1D) * zero.size());
+    UPPE_HIP(hipMemcpyAsync(cb, &C, sizeof C, hipMemcpyHostToDevice, s));
+    UPPE_HIP(hipMemcpyAsync((void *)C.slices, slices, sizeof(Slice1D) * nslices,
+                            hipMemcpyHostToDevice, s));
+    UPPE_HIP(hipMemcpyAsync(C.slice_st, zero.data(), sizeof(Stat1D) * nslices,
+                            hipMemcpyHostToDevice, s));
+    const Chain1D *cd = (const Chain1D *)cb;
+    switch (default_threads(d->nt)) {
+        case 256:
+            if (d->nt <= 2048) uppe1d_chain_kernel<256, 2048><<<1, 256, 0, s>>>(cd);
+            else uppe1d_chain_kernel<256, 8192><<<1, 256, 0, s>>>(cd);
+            break;
+        case 512: uppe1d_chain_kernel<512, 8192><<<1, 512, 0, s>>>(cd); break;
+        default: uppe1d_chain_kernel<1024, 8192><<<1, 1024, 0, s>>>(cd); break;
+    }
+    UPPE_HIP(hipGetLastError());
+    std::vector<Stat1D> hs(nslices);
+    UPPE_HIP(hipMemcpyAsync(hs.data(), C.slice_st, sizeof(Stat1D) * nslices,
+                            hipMemcpyDeviceToHost, s));
+    UPPE_HIP(hipStreamSynchronize(s));
+    UPPE_CHECK(hs[0].attempts >= 0,
+               "uppe1d_propagate_slices: the instance does not fit the launch (nt vs threads)");
+    int bad = 0;
+    for (int i = 0; i < nslices; ++i) {
+        uppe1d_status &q = st[i];
+        const Stat1D &x = hs[i];
+        q.z = x.z;
+        q.h = x.h;
+        q.err = x.err;
+        q.attempts = x.attempts;
+        q.accepted = x.accepted + x.forced;
+        q.rejected = x.rejected;
+        q.forced = x.forced;
+        q.nhist = x.nhist;
+        q.done = x.done;
+        q.nonfinite = x.nonfinite;
+        q.fsal_hits = x.fsal_hits;
+        q.overflow = x.hist_overflow;
+        if (x.nonfinite) bad = 1;
+    }
+    if (bad) {
+        uppe::set_error("uppe1d: error norm became non-finite (NaN/Inf field)");
+        return UPPE_ENONFINITE;
+    }
+    return UPPE_OK;
+}
+
 size_t adj_bytes(int nt, int ckpt_every) {
     const size_t n = (size_t)nt;
     return align256(sizeof(Adj1D)) + align256(5 * (size_t)ckpt_every * n * 8) +
@@ -1395,6 +1672,21 @@ int uppe1d_adjoint(void *fwd_workspace, int n, int index, int M, const double *h
                    double *gU, double *gG, void *stream) {
     return adjoint(fwd_workspace, n, index, M, hs, ckpt, ckpt_every, lam, adj_workspace, gD, gU,
                    gG, (hipStream_t)stream);
+}
+
+int uppe1d_slices_workspace_size(const uppe1d_desc *d, int nslices, size_t *bytes) {
+    UPPE_CHECK(d && bytes && nslices > 0, "uppe1d_slices_workspace_size: bad argument");
+    *bytes = header_bytes(1) + inst_bytes(d->nt) + chain_bytes(d->nt, nslices);
+    return UPPE_OK;
+}
+
+int uppe1d_propagate_slices(const uppe1d_desc *d, int nslices, const uppe1d_slice *slices,
+                            const double *D_re, const double *D_im, const double *uppe_factor,
+                            const double *H_re, const double *H_im, const void *a0,
+                            void *workspace, void *hist, void *zhist, void *out_t, void *out_w,
+                            void *stream, uppe1d_status *st) {
+    return propagate_slices(d, nslices, slices, D_re, D_im, uppe_factor, H_re, H_im, a0,
+                            workspace, hist, zhist, out_t, out_w, (hipStream_t)stream, st);
 }
 
 int uppe1d_propagate(const uppe1d_desc *d, const double *D_re, const double *D_im,

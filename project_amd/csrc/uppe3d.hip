@@ -1453,6 +1453,170 @@ __global__ void state_copy_kernel(const Ctl3D *ctl, Fields3D f, float2 *ext, siz
 }
 
 // ---------------------------------------------------------------------------------------
+// Snapshot diagnostics of the 3+1D MPC runner (src/3+1D-ERK4(3)IP UPPE/sim.py:140-220) as
+// reductions over the state B = FFT_t(A)/a_scale ([omega][x][y]), one pass.  A persistent
+// workgroup takes groups of LINES lines (fixed x, consecutive y; one wave per line),
+// transforms each line back to time (wave_ifft_t) and accumulates in LDS (FP64):
+//   q0[w] = sum |B|^2, q1 + i q2 [w] = sum B m(x, y), q3[t] = sum |A|^2,
+//   q4 + i q5 [t] = sum A m(x, y),  m = exp(-(x^2 + y^2) / w0^2) rounded to float;
+// the per-line fluence sum_t |A|^2 and the centre line's |B|^2, |A|^2 are written directly.
+// Each workgroup writes its accumulators to part[wg][6][NT]; diag_reduce_kernel sums them.
+struct Diag3D {
+    double x0, dx, y0, dy, inv_w02;
+    int ix0, iy0;
+    double *part;      // [gridDim.x][6][NT]
+    double *fluence;   // [nx][ny]
+    double *center;    // [2][NT]: |B(ix0, iy0, w)|^2, |A(ix0, iy0, t)|^2
+};
+
+template <int NT, int LINES>
+__global__ __launch_bounds__(64 * LINES) void diag_kernel(const Ctl3D *ctl, Fields3D f,
+                                                          Tabs3D tb, Diag3D dg, int nx, int ny) {
+    constexpr int R = NT / 64;
+    __shared__ double acc[6 * NT];
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int rl = __brev(lane) >> 26;
+    for (int i = tid; i < 6 * NT; i += 64 * LINES) acc[i] = 0.0;
+    __syncthreads();
+    const float2 *B = ctl->cur ? f.B1 : f.B0;
+    const int plane = nx * ny, bpx = ny / LINES, groups = nx * bpx;
+    for (int g = blockIdx.x; g < groups; g += gridDim.x) {
+        const int x = g / bpx, y = (g % bpx) * LINES + wv;
+        const double xd = dg.x0 + x * dg.dx, yd = dg.y0 + y * dg.dy;
+        const float m = (float)exp(-(xd * xd + yd * yd) * dg.inv_w02);
+        const bool centre = (x == dg.ix0 && y == dg.iy0);
+        float2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int w = rl + 64 * r;
+            v[r] = B[(size_t)w * plane + x * ny + y];
+            const double b2 = (double)v[r].x * v[r].x + (double)v[r].y * v[r].y;
+            atomicAdd(&acc[w], b2);
+            atomicAdd(&acc[NT + w], (double)(v[r].x * m));
+            atomicAdd(&acc[2 * NT + w], (double)(v[r].y * m));
+            if (centre) dg.center[w] = b2;
+        }
+        wave_ifft_t<R>(v, lane, rl, tb.tw_t);   // v[k] = A(t = k + R lane)
+        double fl = 0.0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const int t = k + R * lane;
+            const double a2 = (double)v[k].x * v[k].x + (double)v[k].y * v[k].y;
+            fl += a2;
+            atomicAdd(&acc[3 * NT + t], a2);
+            atomicAdd(&acc[4 * NT + t], (double)(v[k].x * m));
+            atomicAdd(&acc[5 * NT + t], (double)(v[k].y * m));
+            if (centre) dg.center[NT + t] = a2;
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) fl += __shfl_xor(fl, o, 64);
+        if (lane == 0) dg.fluence[x * ny + y] = fl;
+    }
+    __syncthreads();
+    double *out = dg.part + (size_t)blockIdx.x * 6 * NT;
+    for (int i = tid; i < 6 * NT; i += 64 * LINES) out[i] = acc[i];
+}
+
+// out[j] = sum over the nparts partial blocks of part[.][j], j < len (FP64).
+__global__ void diag_reduce_kernel(const double *part, int nparts, int len, double *out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= len) return;
+    double s = 0.0;
+    for (int k = 0; k < nparts; ++k) s += part[(size_t)k * len + j];
+    out[j] = s;
+}
+
+// Fluence moments (sim.py:145-157): sum f, sum x f, sum y f, then the centred second
+// moments sum (x - xm)^2 f, sum (y - ym)^2 f; one workgroup, FP64.
+__global__ __launch_bounds__(1024) void diag_moments_kernel(const double *fl, int nx, int ny,
+                                                            double x0, double dx, double y0,
+                                                            double dy, double *mom) {
+    __shared__ double red[3][16];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int i = tid; i < nx * ny; i += 1024) {
+        const double f = fl[i], x = x0 + (i / ny) * dx, y = y0 + (i % ny) * dy;
+        s[0] += f;
+        s[1] += x * f;
+        s[2] += y * f;
+    }
+    auto block_sum3 = [&](double (&v)[3]) {
+        for (int q = 0; q < 3; ++q) {
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
+        }
+        __syncthreads();
+        if (lane == 0)
+            for (int q = 0; q < 3; ++q) red[q][wv] = v[q];
+        __syncthreads();
+        for (int q = 0; q < 3; ++q) {
+            double t = 0.0;
+            for (int w = 0; w < 16; ++w) t += red[q][w];
+            v[q] = t;
+        }
+    };
+    block_sum3(s);
+    const double tot = s[0] + 1e-30 / (dx * dy);
+    const double xm = s[1] / tot, ym = s[2] / tot;
+    double c[3] = {0.0, 0.0, 0.0};
+    for (int i = tid; i < nx * ny; i += 1024) {
+        const double f = fl[i], x = x0 + (i / ny) * dx - xm, y = y0 + (i % ny) * dy - ym;
+        c[0] += x * x * f;
+        c[1] += y * y * f;
+    }
+    block_sum3(c);
+    if (tid == 0) {
+        mom[0] = s[0];
+        mom[1] = xm;
+        mom[2] = ym;
+        mom[3] = sqrt(c[0] / tot);
+        mom[4] = sqrt(c[1] / tot);
+    }
+}
+
+// out[x][y] = A(x, y, t = it) / a_scale = (1/NT) sum_w B(w, x, y) exp(+2 pi i w it / NT).
+__global__ void field_at_t_kernel(const Ctl3D *ctl, Fields3D f, Tabs3D tb, int nx, int ny,
+                                  int nt, int it, float2 *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nx * ny) return;
+    const float2 *B = ctl->cur ? f.B1 : f.B0;
+    double re = 0.0, im = 0.0;
+    for (int w = 0; w < nt; ++w) {
+        const float2 b = B[(size_t)w * nx * ny + i];
+        const float2 e = tb.tw_t[(int)(((long long)w * it) % nt)];   // exp(-2 pi i w it / nt)
+        re += (double)b.x * e.x + (double)b.y * e.y;                   // b * conj(e)
+        im += (double)b.y * e.x - (double)b.x * e.y;
+    }
+    out[i] = make_float2((float)(re / nt), (float)(im / nt));
+}
+
+// Curved mirror of sim.py:82-96 on the state (diagonal in [omega][x][y]):
+// B *= amp exp(-i (omega0 + omega) r^2 / (c R)); R = 0: no curvature (amp only).
+__global__ void mirror_kernel(const Ctl3D *ctl, Fields3D f, int nx, int ny, int nt,
+                              double omega0, double dt, double x0, double dx, double y0,
+                              double dy, double R, double amp) {
+    float2 *B = ctl->cur ? f.B1 : f.B0;
+    const size_t n = (size_t)nx * ny * nt;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int w = (int)(i / ((size_t)nx * ny));
+        const int xy = (int)(i % ((size_t)nx * ny));
+        const double x = x0 + (xy / ny) * dx, y = y0 + (xy % ny) * dy;
+        const int k = w < (nt + 1) / 2 ? w : w - nt;              // fftfreq order
+        const double om = 2.0 * M_PI * (double)k / ((double)nt * dt);
+        double c = amp, sn = 0.0;
+        if (R != 0.0) {
+            double sv, cv;
+            sincos(-(omega0 + om) / 299792458.0 * (x * x + y * y) / R, &sv, &cv);
+            c = amp * cv;
+            sn = amp * sv;
+        }
+        const float2 b = B[i];
+        B[i] = make_float2((float)(b.x * c - b.y * sn), (float)(b.x * sn + b.y * c));
+    }
+}
+
+// ---------------------------------------------------------------------------------------
 // Planes larger than one workgroup's LDS (256 x 256, 512 KB): the 2D transform is split into
 // a column pass (length-256 transforms along x, stride Ny) and a row kernel (transform along
 // y, the stage arithmetic of one xy pass in (kx, ky), inverse transform along y); a second
@@ -1612,7 +1776,7 @@ __global__ void a5pre_kernel(Adj3D a, Tabs3D tb, float h, int plane, size_t n) {
 // ---------------------------------------------------------------------------------------
 // Step controller (solver_3d.py:182-208), one thread, FP64.  Also applies the top-of-loop
 // clip h = min(h, L - z) of the next iteration and the h < 1e-15 stop.
-__global__ void ctl_kernel(Ctl3D *c) {
+__device__ void controller(Ctl3D *c) {
     for (int i = 0; i < 8; ++i) c->plasma_groups[i] = 0;
     if (c->done) return;
     const double h = c->h;
@@ -1662,6 +1826,19 @@ __global__ void ctl_kernel(Ctl3D *c) {
         if (hn < 1e-15) c->done = 1;
     }
     c->h = hn;
+}
+
+// End of an attempt: the controller (thread 0), then the per-plane FP64 phases of the new h
+// (what phase_kernel computes) by the whole workgroup, in one launch.
+__global__ __launch_bounds__(1024) void ctl_phase_kernel(Ctl3D *c, Tabs3D tb, int nt) {
+    if (threadIdx.x == 0) controller(c);
+    __syncthreads();
+    if (c->done) return;
+    const double s = 0.5 * c->h;
+    for (int w = threadIdx.x; w < nt; w += blockDim.x) {
+        tb.ph_d[w] = cis_d(s * tb.d64[w]);
+        tb.ph_r[w] = cis_d(-s * tb.kref64[w]);
+    }
 }
 
 __global__ void begin_kernel(Ctl3D *c, double L, double tol, double max_step, double min_step) {
@@ -1945,8 +2122,7 @@ int enqueue_attempt(uppe3d_plan *p, hipStream_t s, hipEvent_t *ev = nullptr) {
     RET(mark());
     RET(launch_xy(p, 6, s));                          // P6
     RET(mark());
-    ctl_kernel<<<1, 1, 0, s>>>(p->ctl);
-    phase_kernel<<<(p->d.nt + 255) / 256, 256, 0, s>>>(p->ctl, p->tb, p->d.nt);
+    ctl_phase_kernel<<<1, 1024, 0, s>>>(p->ctl, p->tb, p->d.nt);
     UPPE_HIP(hipGetLastError());
     RET(mark());
     return UPPE_OK;
@@ -2301,6 +2477,79 @@ int uppe3d_get_state(uppe3d_plan *p, void *dst, void *stream) {
     UPPE_CHECK(p && dst, "uppe3d_get_state: null argument");
     const size_t n = (size_t)p->d.nx * p->d.ny * p->d.nt;
     state_copy_kernel<<<1024, 256, 0, (hipStream_t)stream>>>(p->ctl, p->f, (float2 *)dst, n, 0);
+    UPPE_HIP(hipGetLastError());
+    return UPPE_OK;
+}
+
+static int diag_parts(const uppe3d_desc &d) {
+    const int lines = d.nt >= 1024 ? 8 : 16;   // as the DIAG launches below
+    const int groups = d.nx * (d.ny / lines);
+    return groups < 256 ? groups : 256;
+}
+
+int uppe3d_diag_workspace_size(const uppe3d_desc *d, size_t *bytes) {
+    RET(check_desc(d));
+    UPPE_CHECK(bytes != nullptr, "uppe3d_diag_workspace_size: null argument");
+    *bytes = sizeof(double) * 6 * (size_t)d->nt * diag_parts(*d);
+    return UPPE_OK;
+}
+
+int uppe3d_diagnostics(uppe3d_plan *p, double w0_mode, double x0, double dx, double y0,
+                       double dy, int ix0, int iy0, void *work, void *out, void *stream) {
+    UPPE_CHECK(p && work && out, "uppe3d_diagnostics: null argument");
+    UPPE_CHECK(w0_mode > 0.0 && dx > 0.0 && dy > 0.0, "uppe3d_diagnostics: bad geometry");
+    hipStream_t s = (hipStream_t)stream;
+    const int nx = p->d.nx, ny = p->d.ny, nt = p->d.nt, G = diag_parts(p->d);
+    double *o = (double *)out;
+    Diag3D dg;
+    dg.x0 = x0;
+    dg.dx = dx;
+    dg.y0 = y0;
+    dg.dy = dy;
+    dg.inv_w02 = 1.0 / (w0_mode * w0_mode);
+    dg.ix0 = ix0;
+    dg.iy0 = iy0;
+    dg.part = (double *)work;
+    dg.center = o + 6 * (size_t)nt;
+    dg.fluence = o + 8 * (size_t)nt;
+    switch (nt) {
+#define DIAG(NT, LN)                                                                     \
+    case NT:                                                                             \
+        diag_kernel<NT, LN><<<G, 64 * LN, 0, s>>>(p->ctl, p->f, p->tb, dg, nx, ny);      \
+        break;
+        DIAG(128, 16)
+        DIAG(256, 16)
+        DIAG(512, 16)
+        DIAG(1024, 8)
+        DIAG(2048, 8)
+#undef DIAG
+        default:
+            return UPPE_EINVAL;
+    }
+    UPPE_HIP(hipGetLastError());
+    diag_reduce_kernel<<<(6 * nt + 255) / 256, 256, 0, s>>>(dg.part, G, 6 * nt, o);
+    diag_moments_kernel<<<1, 1024, 0, s>>>(dg.fluence, nx, ny, x0, dx, y0, dy,
+                                           o + 8 * (size_t)nt + (size_t)nx * ny);
+    UPPE_HIP(hipGetLastError());
+    return UPPE_OK;
+}
+
+int uppe3d_field_at_time(uppe3d_plan *p, int it, void *out_xy, void *stream) {
+    UPPE_CHECK(p && out_xy, "uppe3d_field_at_time: null argument");
+    UPPE_CHECK(it >= 0 && it < p->d.nt, "uppe3d_field_at_time: time index out of range");
+    const int n = p->d.nx * p->d.ny;
+    field_at_t_kernel<<<(n + 255) / 256, 256, 0, (hipStream_t)stream>>>(
+        p->ctl, p->f, p->tb, p->d.nx, p->d.ny, p->d.nt, it, (float2 *)out_xy);
+    UPPE_HIP(hipGetLastError());
+    return UPPE_OK;
+}
+
+int uppe3d_apply_mirror(uppe3d_plan *p, double R, double amp, double x0, double dx, double y0,
+                        double dy, void *stream) {
+    UPPE_CHECK(p != nullptr, "uppe3d_apply_mirror: null plan");
+    mirror_kernel<<<1024, 256, 0, (hipStream_t)stream>>>(p->ctl, p->f, p->d.nx, p->d.ny, p->d.nt,
+                                                         p->d.omega0, p->d.dt, x0, dx, y0, dy,
+                                                         R, amp);
     UPPE_HIP(hipGetLastError());
     return UPPE_OK;
 }

@@ -235,6 +235,86 @@ class ERK43IP_UPPE:
         A_hist = hist[:nh].cpu().numpy().astype(np.complex128)
         return z_hist, A_hist, omega
 
+    def propagate_slices(self, A0, t, lengths, gammas=None, tol=1e-4, max_step=1e-2,
+                         min_step=1e-11, record="all", max_hist=None):
+        """A chain of propagations in one kernel launch (the slicing loop of gas.py:113-147 and
+        "gas copy.py":115-141, with no host round trip between slices).
+
+        Slice i propagates over lengths[i] with nonlinear coefficient gammas[i] (default
+        self.gamma) and its own tol / max_step / min_step (scalars or one value per slice),
+        starting like a fresh ``propagate`` call from the field the reference loop hands on:
+        A_hist[-1] of slice i-1.  record="all": returns (z_hists, A_hists, omega), one
+        (z_hist, A_hist) per slice as ``propagate`` returns them; record="last": returns
+        (A_out, omega), A_out (n, Nt) complex128 the field each slice hands on.
+        max_hist: history snapshots kept in total for record="all" (default: up to
+        1502 per slice within 1 GiB); running out raises.
+        """
+        import torch
+        lib = _lib.load()
+        if record not in ("all", "last"):
+            raise ValueError("record must be 'all' or 'last'")
+        t = np.asarray(t, np.float64)
+        N = len(t)
+        dt = t[1] - t[0]
+        omega = 2 * np.pi * fftfreq(N, dt)
+        lengths = np.atleast_1d(np.asarray(lengths, np.float64))
+        n = len(lengths)
+        if n == 0:
+            raise ValueError("no slices")
+        per = lambda v: np.broadcast_to(np.asarray(v, np.float64), (n,))
+        g = per(self.gamma if gammas is None else gammas)
+        tl, mx, mn = per(tol), per(max_step), per(min_step)
+        slices = (_lib.Uppe1DSlice * n)()
+        for i in range(n):
+            slices[i] = _lib.Uppe1DSlice(L=lengths[i], gamma=g[i], tol=tl[i], max_step=mx[i],
+                                         min_step=mn[i])
+        D, uppe, H = self._tables(omega)
+        dev = torch.device("cuda")
+        if record == "all":
+            if max_hist is None:
+                max_hist = int(min(n * self.MAX_HIST, max(64 * n, (1 << 30) // (8 * N))))
+        else:
+            max_hist = 0
+        d = self._desc(N, dt, 0.0, 1.0, 1.0, 0.0, max_hist)
+        nbytes = ctypes.c_size_t()
+        _lib.check(lib.uppe1d_slices_workspace_size(ctypes.byref(d), n, ctypes.byref(nbytes)),
+                   "uppe1d_slices_workspace_size")
+        ws = torch.empty(nbytes.value, dtype=torch.uint8, device=dev)
+        a0 = torch.from_numpy(np.ascontiguousarray(A0, np.complex64)).to(dev)
+        hist = torch.empty((max(max_hist, 1), N), dtype=torch.complex64, device=dev)
+        zh = torch.empty(max(max_hist, 1), dtype=torch.float64, device=dev)
+        out_t = torch.empty((n, N), dtype=torch.complex64, device=dev) if record == "last" \
+            else None
+        Dre, Dim = np.ascontiguousarray(D.real), np.ascontiguousarray(D.imag)
+        uf = np.ascontiguousarray(uppe, np.float64)
+        Hre = np.ascontiguousarray(H.real) if H is not None else None
+        Him = np.ascontiguousarray(H.imag) if H is not None else None
+        st = (_lib.Uppe1DStatus * n)()
+        rc = lib.uppe1d_propagate_slices(
+            ctypes.byref(d), n, slices, _lib.host_ptr(Dre), _lib.host_ptr(Dim),
+            _lib.host_ptr(uf), _lib.host_ptr(Hre) if Hre is not None else None,
+            _lib.host_ptr(Him) if Him is not None else None, _lib.ptr(a0), _lib.ptr(ws),
+            _lib.ptr(hist) if max_hist > 0 else None, _lib.ptr(zh) if max_hist > 0 else None,
+            _lib.ptr(out_t) if out_t is not None else None, None, _lib.stream_handle(), st)
+        if rc == -3:
+            raise FloatingPointError(lib.uppe_last_error().decode())
+        _lib.check(rc, "uppe1d_propagate_slices")
+        self.last_slice_status = list(st)
+        self.last_status = st[n - 1]
+        if record == "last":
+            return out_t.cpu().numpy().astype(np.complex128), omega
+        if any(x.overflow & 1 for x in st):
+            raise RuntimeError("propagate_slices: history buffer of %d snapshots ran out; pass a "
+                               "larger max_hist" % max_hist)
+        counts = [x.nhist for x in st]
+        tot = sum(counts)
+        zall = zh[:tot].cpu().numpy()
+        Aall = hist[:tot].cpu().numpy().astype(np.complex128)
+        edges = np.concatenate([[0], np.cumsum(counts)])
+        z_hists = [zall[edges[i]:edges[i + 1]] for i in range(n)]
+        A_hists = [Aall[edges[i]:edges[i + 1]] for i in range(n)]
+        return z_hists, A_hists, omega
+
     # -- input generators (host; erk43ip_method.py:375-446) ----------------------------------
     def generate_gaussian_pulse(self, t, pulse_energy, pulse_fwhm, chirp=0):
         P0 = pulse_energy / (pulse_fwhm * np.sqrt(np.pi / (4 * np.log(2))))
@@ -250,6 +330,30 @@ class ERK43IP_UPPE:
         omega = 2 * np.pi * fftfreq(len(t), t[1] - t[0])
         phase = 0.5 * GDD * omega ** 2 + (1.0 / 6.0) * TOD * omega ** 3
         return fftshift(ifft(fft(ifftshift(A_tl)) * np.exp(1j * phase)))
+
+
+def _attach_spectrum_methods():
+    from . import spectrum_io
+
+    def generate_pulse_from_spectrum(self, t, wavelengths_nm, intensities, pulse_energy, GDD=0,
+                                     TOD=0, use_jacobian=True, interp_kind='cubic',
+                                     exact_fourier=False, time_window_auto=True,
+                                     N_time_points=4096):
+        """Pulse from a measured spectrum (erk43ip_method.py:599-764); host FP64, see
+        project_amd.spectrum_io.pulse_from_spectrum."""
+        return spectrum_io.pulse_from_spectrum(
+            t, wavelengths_nm, intensities, pulse_energy, self.lambda0, GDD=GDD, TOD=TOD,
+            use_jacobian=use_jacobian, interp_kind=interp_kind, exact_fourier=exact_fourier,
+            time_window_auto=time_window_auto, N_time_points=N_time_points, c=self.c)
+
+    def _calculate_fwhm_precise(self, x, y):
+        return spectrum_io.fwhm_precise(x, y)
+
+    ERK43IP_UPPE.generate_pulse_from_spectrum = generate_pulse_from_spectrum
+    ERK43IP_UPPE._calculate_fwhm_precise = _calculate_fwhm_precise
+
+
+_attach_spectrum_methods()
 
 
 class SimResultAdapter:

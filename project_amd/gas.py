@@ -10,8 +10,11 @@ reference patches its class at import time:
   ``material == 'argon_gas'``; other materials keep the Sellmeier table.
 
 The formula's constants are the published Peck & Fisher values the reference uses.
+``calculate_cavity_mode`` (gas.py:49-57) is re-exported from :mod:`project_amd.cavity`, which
+also holds the slicing drivers of gas.py's ``main``.
 """
 from .erk43ip_method import ERK43IP_UPPE, SimResultAdapter  # noqa: F401
+from .cavity import calculate_cavity_mode  # noqa: F401  (gas.py:49-57)
 
 _orig_compute_ri = ERK43IP_UPPE._compute_refractive_index
 _orig_init = ERK43IP_UPPE.__init__

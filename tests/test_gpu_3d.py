@@ -41,6 +41,11 @@ CASES = [
     ("32x256 plasma stress", 32, 256, 30e-3, False, True, 0.001, 1e-4, 1e-3),
     # the longest supported lines (NT = 2048, 32 values per lane)
     ("16x2048 kerr+hoke", 16, 2048, 0.8e-3, True, False, 0.02, 1e-5, 2e-5),
+    # NT = 1024 (two-team t-pass): one 16-line tile per workgroup; groups deferred to the
+    # 8-line plasma instantiation (scan and serial recurrence, with shock)
+    ("16x1024 kerr+hoke", 16, 1024, 0.8e-3, True, False, 0.02, 1e-5, 2e-5),
+    ("32x1024 plasma scan+shock", 32, 1024, 4e-3, False, True, 0.005, 1e-5, 2e-5),
+    ("32x1024 plasma stress", 32, 1024, 30e-3, False, True, 0.001, 1e-4, 1e-3),
     # planes beyond one workgroup's LDS (column pass + row kernel)
     ("256x128 kerr+hoke+shock", 256, 128, 0.8e-3, True, True, 0.02, 1e-5, 2e-5),
 ]
