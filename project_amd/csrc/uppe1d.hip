@@ -56,44 +56,66 @@ struct Inst1D {
     Stat1D *st;
 };
 
-// Stockham stage of radix R over LDS: src -> dst.
-template <int T, int R, int DIR>
-__device__ __forceinline__ void lds_stage(const float2 *src, float2 *dst, int n, int ns,
-                                          const float2 *__restrict__ tw, int tid) {
-    const int nb = n / R;
-    for (int j = tid; j < nb; j += T) {
-        float2 v[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) v[r] = src[j + r * nb];
-        const int b = stockham_butterfly<R, DIR>(v, j, ns, n, tw);
-#pragma unroll
-        for (int r = 0; r < R; ++r) dst[b + r * ns] = v[r];
-    }
+// Length-N FFT (N a compile-time power of two) of the data in a (natural order) through LDS:
+// Stockham autosort stages of radix 8 (then 4 or 2 for the remainder), stage size NS a
+// template constant, so the butterfly index arithmetic is shifts and masks and the stage loop
+// is unrolled.  Input x[j + r N/R] -> output y[(j/NS) NS R + j%NS + r NS].  Returns the buffer
+// holding the result.
+// Returns p unchanged, hidden from the optimiser: twiddle loads through the result stay in
+// their stage instead of being hoisted above the barriers of earlier stages.
+__device__ __forceinline__ const float2 *opaque_tw(const float2 *p) {
+    asm volatile("" : "+v"(p));
+    return p;
 }
 
-// Length-n FFT of the data in a (natural order); returns the buffer holding the result.
-template <int T, int DIR>
-__device__ float2 *lds_fft(float2 *a, float2 *b, int n, const float2 *__restrict__ tw, int tid) {
-    int ns = 1;
-    float2 *src = a, *dst = b;
-    while (ns < n) {
-        const int rem = n / ns;
-        if (rem >= 8) {
-            lds_stage<T, 8, DIR>(src, dst, n, ns, tw, tid);
-            ns *= 8;
-        } else if (rem == 4) {
-            lds_stage<T, 4, DIR>(src, dst, n, ns, tw, tid);
-            ns *= 4;
-        } else {
-            lds_stage<T, 2, DIR>(src, dst, n, ns, tw, tid);
-            ns *= 2;
+template <int T, int N, int NS, int DIR>
+struct LdsFFT {
+    __device__ __forceinline__ static float2 *run(float2 *src, float2 *dst,
+                                                  const float2 *__restrict__ tw0, int tid) {
+        const float2 *tw = opaque_tw(tw0);
+        constexpr int REM = N / NS;
+        constexpr int R = REM >= 8 ? 8 : (REM == 4 ? 4 : 2);
+        constexpr int NB = N / R;
+        constexpr int STEP = N / (NS * R);   // twiddle stride per unit of k
+#pragma unroll
+        for (int j0 = 0; j0 < NB; j0 += T) {
+            const int j = j0 + tid;
+            if (NB % T == 0 || j < NB) {
+                float2 v[R];
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[r] = src[j + r * NB];
+                const int k = j & (NS - 1);
+                if constexpr (NS > 1) {
+#pragma unroll
+                    for (int r = 1; r < R; ++r) {
+                        const float2 w = tw[k * STEP * r];
+                        v[r] = DIR > 0 ? cmul(v[r], w) : cmulc(v[r], w);
+                    }
+                }
+                Dft<R, DIR>::run(v);
+                const int b = (j - k) * R + k;   // (j / NS) NS R + j % NS
+#pragma unroll
+                for (int r = 0; r < R; ++r) dst[b + r * NS] = v[r];
+            }
         }
         __syncthreads();
-        float2 *t = src;
-        src = dst;
-        dst = t;
+        return LdsFFT<T, N, NS * R, DIR>::run(dst, src, tw0, tid);
     }
-    return src;
+};
+template <int T, int N, int DIR>
+struct LdsFFT<T, N, N, DIR> {
+    __device__ __forceinline__ static float2 *run(float2 *src, float2 *, const float2 *, int) {
+        return src;
+    }
+};
+
+// Out of line: the callers hold large live state (stage fields), the FFT gets its own
+// register allocation.
+template <int T, int DIR, int N>
+__device__ __noinline__ float2 *lds_fft_c(float2 *a, float2 *b, const float2 *__restrict__ tw,
+                                          int tid) {
+    static_assert((N & (N - 1)) == 0 && N >= 2, "power-of-two length");
+    return LdsFFT<T, N, 1, DIR>::run(a, b, tw, tid);
 }
 
 template <int T>
@@ -262,7 +284,7 @@ __device__ void neval(const Inst1D &P, const float2 (&F)[E], float2 (&out)[E], f
     for (int i = 0; i < E; ++i)
         if (i < E) bufA[tid + T * i] = F[i];
     __syncthreads();
-    float2 *r = lds_fft<T, -1>(bufA, bufB, n, P.tw, tid);
+    float2 *r = lds_fft_c<T, -1, n>(bufA, bufB, P.tw, tid);
     const float invn = 1.0f / (float)n;
     float2 a[E], Ie[E];
 #pragma unroll
@@ -279,14 +301,14 @@ __device__ void neval(const Inst1D &P, const float2 (&F)[E], float2 (&out)[E], f
         for (int i = 0; i < E; ++i)
             if (i < E) bufA[tid + T * i] = Ie[i];
         __syncthreads();
-        float2 *q = lds_fft<T, 1>(bufA, bufB, n, P.tw, tid);
+        float2 *q = lds_fft_c<T, 1, n>(bufA, bufB, P.tw, tid);
         float2 *o = (q == bufA) ? bufB : bufA;
 #pragma unroll
         for (int i = 0; i < E; ++i)
             if (i < E) o[tid + T * i] = cmul(q[tid + T * i], P.H[tid + T * i]);
         __syncthreads();
         float2 *o2 = (o == bufA) ? bufB : bufA;
-        float2 *d = lds_fft<T, -1>(o, o2, n, P.tw, tid);
+        float2 *d = lds_fft_c<T, -1, n>(o, o2, P.tw, tid);
         const float fr = (float)P.f_R;
 #pragma unroll
         for (int i = 0; i < E; ++i)
@@ -326,7 +348,7 @@ __device__ void neval(const Inst1D &P, const float2 (&F)[E], float2 (&out)[E], f
         for (int i = 0; i < E; ++i)
             if (i < E) bufA[tid + T * i] = cmul(a[i], Ie[i]);
         __syncthreads();
-        float2 *k = lds_fft<T, 1>(bufA, bufB, n, P.tw, tid);
+        float2 *k = lds_fft_c<T, 1, n>(bufA, bufB, P.tw, tid);
 #pragma unroll
         for (int i = 0; i < E; ++i)
             if (i < E) {
@@ -343,7 +365,7 @@ __device__ void neval(const Inst1D &P, const float2 (&F)[E], float2 (&out)[E], f
                     bufA[tid + T * i] = make_float2(-c1[i] * a[i].x + c2[i] * a[i].y,
                                                     -c1[i] * a[i].y - c2[i] * a[i].x);
             __syncthreads();
-            float2 *q = lds_fft<T, 1>(bufA, bufB, n, P.tw, tid);
+            float2 *q = lds_fft_c<T, 1, n>(bufA, bufB, P.tw, tid);
 #pragma unroll
             for (int i = 0; i < E; ++i)
                 if (i < E) out[i] = cadd(out[i], q[tid + T * i]);
@@ -359,7 +381,7 @@ __device__ void neval(const Inst1D &P, const float2 (&F)[E], float2 (&out)[E], f
                                                 g * p.x - c1[i] * a[i].y - c2[i] * a[i].x);
             }
         __syncthreads();
-        float2 *k = lds_fft<T, 1>(bufA, bufB, n, P.tw, tid);
+        float2 *k = lds_fft_c<T, 1, n>(bufA, bufB, P.tw, tid);
 #pragma unroll
         for (int i = 0; i < E; ++i)
             if (i < E) out[i] = k[tid + T * i];
@@ -446,7 +468,7 @@ __device__ void run_instance(const Inst1D &P, float2 *bufA, float2 *bufB, double
         }
     __syncthreads();
     {
-        float2 *r = lds_fft<T, 1>(bufA, bufB, n, P.tw, tid);
+        float2 *r = lds_fft_c<T, 1, n>(bufA, bufB, P.tw, tid);
         float2 A[E], N0[E];
 #pragma unroll
         for (int i = 0; i < E; ++i)
@@ -659,7 +681,7 @@ __device__ void run_chain(const Chain1D &C, float2 *bufA, float2 *bufB, double *
             for (int i = 0; i < E; ++i)
                 if (i < E) bufA[tid + T * i] = Q.a0[tid + T * i];
             __syncthreads();
-            float2 *r = lds_fft<T, 1>(bufA, bufB, n, P.tw, tid);
+            float2 *r = lds_fft_c<T, 1, n>(bufA, bufB, P.tw, tid);
 #pragma unroll
             for (int i = 0; i < E; ++i)
                 if (i < E) A[i] = r[tid + T * i];
@@ -775,7 +797,7 @@ __device__ void run_chain(const Chain1D &C, float2 *bufA, float2 *bufB, double *
             for (int i = 0; i < E; ++i)
                 if (i < E) bufA[tid + T * i] = C.lastrec[tid + T * i];
             __syncthreads();
-            float2 *r = lds_fft<T, -1>(bufA, bufB, n, P.tw, tid);
+            float2 *r = lds_fft_c<T, -1, n>(bufA, bufB, P.tw, tid);
             const float invn = 1.0f / (float)n;
 #pragma unroll
             for (int i = 0; i < E; ++i)
@@ -987,7 +1009,7 @@ __device__ void adj_neval(const Adj1D &J, const float2 (&F)[E], const float2 (&y
     for (int i = 0; i < E; ++i)
         if (i < E) bufA[tid + T * i] = F[i];
     __syncthreads();
-    float2 *r = lds_fft<T, -1>(bufA, bufB, n, P.tw, tid);
+    float2 *r = lds_fft_c<T, -1, n>(bufA, bufB, P.tw, tid);
     float2 a[E], pb[E], Pb[E], ab[E];
 #pragma unroll
     for (int i = 0; i < E; ++i)
@@ -999,7 +1021,7 @@ __device__ void adj_neval(const Adj1D &J, const float2 (&F)[E], const float2 (&y
         for (int i = 0; i < E; ++i)
             if (i < E) bufA[tid + T * i] = cscale(a[i], cabs2(a[i]));
         __syncthreads();
-        float2 *q = lds_fft<T, 1>(bufA, bufB, n, P.tw, tid);
+        float2 *q = lds_fft_c<T, 1, n>(bufA, bufB, P.tw, tid);
         float2 Qb[E];
 #pragma unroll
         for (int i = 0; i < E; ++i)
@@ -1017,7 +1039,7 @@ __device__ void adj_neval(const Adj1D &J, const float2 (&F)[E], const float2 (&y
         for (int i = 0; i < E; ++i)
             if (i < E) bufA[tid + T * i] = Qb[i];
         __syncthreads();
-        float2 *r2 = lds_fft<T, -1>(bufA, bufB, n, P.tw, tid);   // n IFFT
+        float2 *r2 = lds_fft_c<T, -1, n>(bufA, bufB, P.tw, tid);   // n IFFT
 #pragma unroll
         for (int i = 0; i < E; ++i)
             if (i < E) pb[i] = r2[tid + T * i];
@@ -1027,7 +1049,7 @@ __device__ void adj_neval(const Adj1D &J, const float2 (&F)[E], const float2 (&y
             for (int i = 0; i < E; ++i)
                 if (i < E) bufA[tid + T * i] = yb[i];
             __syncthreads();
-            float2 *r3 = lds_fft<T, -1>(bufA, bufB, n, P.tw, tid);
+            float2 *r3 = lds_fft_c<T, -1, n>(bufA, bufB, P.tw, tid);
 #pragma unroll
             for (int i = 0; i < E; ++i)
                 if (i < E) Pb[i] = r3[tid + T * i];
@@ -1038,7 +1060,7 @@ __device__ void adj_neval(const Adj1D &J, const float2 (&F)[E], const float2 (&y
         for (int i = 0; i < E; ++i)
             if (i < E) bufA[tid + T * i] = yb[i];
         __syncthreads();
-        float2 *r2 = lds_fft<T, -1>(bufA, bufB, n, P.tw, tid);   // wb = n IFFT(yb)
+        float2 *r2 = lds_fft_c<T, -1, n>(bufA, bufB, P.tw, tid);   // wb = n IFFT(yb)
 #pragma unroll
         for (int i = 0; i < E; ++i)
             if (i < E) {
@@ -1064,7 +1086,7 @@ __device__ void adj_neval(const Adj1D &J, const float2 (&F)[E], const float2 (&y
     for (int i = 0; i < E; ++i)
         if (i < E) bufA[tid + T * i] = ab[i];
     __syncthreads();
-    float2 *q2 = lds_fft<T, 1>(bufA, bufB, n, P.tw, tid);
+    float2 *q2 = lds_fft_c<T, 1, n>(bufA, bufB, P.tw, tid);
 #pragma unroll
     for (int i = 0; i < E; ++i)
         if (i < E) Fb[i] = cscale(q2[tid + T * i], invn);
