@@ -130,6 +130,16 @@ int uppe3d_field_at_time(uppe3d_plan *p, int it, void *out_xy, void *stream);
 int uppe3d_apply_mirror(uppe3d_plan *p, double R, double amp, double x0, double dx, double y0,
                         double dy, void *stream);
 
+/* Error norm of the embedded estimate for every following attempt of the plan.
+ * UPPE_NORM_LINF (the default of a new plan): max|A4 - A3| / (max|A4| + 1e-20), replaces the
+ * envelope solver's norm at src/3+1D-UPPE-ERK4(3)IP-envelope/solver_3d.py:151-153.
+ * UPPE_NORM_L2: sqrt(sum|A4 - A3|^2) / (sqrt(sum|A4|^2) + 1e-20) over the whole
+ * (kx, ky, omega) array, the norm of the MPC runner's solver,
+ * src/3+1D-ERK4(3)IP UPPE/solver_3d.py:252-254 (sums in FP64 over FP32 squares). */
+#define UPPE_NORM_LINF 0
+#define UPPE_NORM_L2 1
+int uppe3d_set_error_norm(uppe3d_plan *p, int norm);
+
 /* Adjoint of the 3+1D step (reverse mode through a frozen step sequence; no counterpart in
  * the reference).  uppe3d_set_stage_record makes every following attempt copy its stage
  * fields X2, X3, X4 (the inputs of the 2nd..4th nonlinear evaluations, same layout as the

@@ -116,7 +116,7 @@ class GasMedium:
         return kerr, loss + defo
 
 
-def dispersion_operator(medium, omega, kperp2):
+def dispersion_operator(medium, omega, kperp2, valid_mask=True):
     """D(x-index, y-index, omega) of solver_3d.py:56-87 (FP64)."""
     w0 = medium.omega0
     n_w = medium.refractive_index(omega + w0)
@@ -131,10 +131,18 @@ def dispersion_operator(medium, omega, kperp2):
     kp = kperp2[:, :, None]
     kz = np.sqrt((ksq - kp).astype(np.complex128))
     D = 1j * (((ksq - kref[None, None, :] ** 2) - kp) / (kz + kref[None, None, :]))
-    return D * (omega + w0 > 0)[None, None, :], k, kref, k0, k1
+    if valid_mask:
+        D = D * (omega + w0 > 0)[None, None, :]
+    return D, k, kref, k0, k1
 
 
 class Oracle3D:
+    # error norm of the embedded estimate: "linf" (solver_3d.py:151-153) or "l2" (the older
+    # solver of src/3+1D-ERK4(3)IP UPPE/solver_3d.py:252-254); valid_mask False drops the
+    # negative-absolute-frequency mask from D and N (that older solver has none)
+    err_norm = "linf"
+    valid_mask = True
+
     def __init__(self, x, y, t, medium, workers=None):
         self.m = medium
         self.workers = workers
@@ -148,7 +156,7 @@ class Oracle3D:
         self.omega = 2 * np.pi * sfft.fftfreq(self.Nt, self.dt)
         self.freq_axis_hz = sfft.fftshift(self.omega) / (2 * np.pi)
         self.D, self.k, self.kref, self.k0, self.k1 = dispersion_operator(
-            medium, self.omega, self.kperp2)
+            medium, self.omega, self.kperp2, self.valid_mask)
         X, Y = np.meshgrid(self.x, self.y, indexing="ij")
         R = np.sqrt(X ** 2 + Y ** 2)
         hw = float(np.max(np.abs(self.x)))
@@ -159,6 +167,8 @@ class Oracle3D:
         fmax = float(np.max(np.abs(f))) + 1e-10
         self.filt = np.exp(-(f / (0.85 * fmax)) ** 16)[None, None, :]
         self.mask = (self.omega + medium.omega0 > 0).astype(np.float64)[None, None, :]
+        if not self.valid_mask:
+            self.mask = np.ones_like(self.mask)
         self.A_scale = 1.0
 
     # FFT helpers (backward norm, as scipy/cupyx)
@@ -188,7 +198,11 @@ class Oracle3D:
         A4 = beta + (h / 6) * a4
         a5 = self.N(A4)
         A3 = beta + (h / 30) * (2 * a4 + 3 * a5)
-        err = float(np.max(np.abs(A4 - A3)) / (np.max(np.abs(A4)) + 1e-20))
+        if self.err_norm == "l2":
+            err = float(np.sqrt(np.sum(np.abs(A4 - A3) ** 2)) /
+                        (np.sqrt(np.sum(np.abs(A4) ** 2)) + 1e-20))
+        else:
+            err = float(np.max(np.abs(A4 - A3)) / (np.max(np.abs(A4)) + 1e-20))
         A4w = self._fi(A4, (0, 1)) * self.filt
         A4t = self._fi(A4w, (2,)) * np.exp(-self.damp * h)
         return A4t, err

@@ -12,6 +12,8 @@ LIB_PATH = os.path.join(HERE, "libuppe.so")
 c_int, c_double, c_void_p, c_size_t = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_size_t
 P = ctypes.POINTER
 
+UPPE_NORM_LINF, UPPE_NORM_L2 = 0, 1   # uppe3d_set_error_norm
+
 
 class Uppe3DDesc(ctypes.Structure):
     _fields_ = [("nx", c_int), ("ny", c_int), ("nt", c_int), ("dt", c_double),
@@ -82,6 +84,7 @@ SIGNATURES = {
     "uppe3d_field_at_time": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     "uppe3d_apply_mirror": (c_int, [c_void_p, c_double, c_double, c_double, c_double, c_double,
                                     c_double, c_void_p]),
+    "uppe3d_set_error_norm": (c_int, [c_void_p, c_int]),
     "uppe3d_set_stage_record": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "uppe3d_adjoint_workspace_size": (c_int, [P(Uppe3DDesc), P(c_size_t)]),
     "uppe3d_adjoint_step": (c_int, [c_void_p, c_double] + [c_void_p] * 7),

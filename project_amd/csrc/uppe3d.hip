@@ -23,7 +23,9 @@
 //   P5: A4 = Q + h/6 F(Y4); max|A4|^2; X = F^-1(A4)
 //   T5: D = Y4 - N_M(X); B' = X * filter * exp(-damp h)
 //   P6: max|F(D)|^2;  err = (h/10) max|a4' - a5'| / (max|A4| + 1e-20)  = the reference's
-//       max|A4 - A3| / (max|A4| + 1e-20), since A4 - A3 = (h/10)(a4' - a5')
+//       max|A4 - A3| / (max|A4| + 1e-20), since A4 - A3 = (h/10)(a4' - a5').  With the L2
+//       norm (uppe3d_set_error_norm) P5 and P6 also sum the squares (FP64 per workgroup) and
+//       err = (h/10) sqrt(sum|F(D)|^2) / (sqrt(sum|A4|^2) + 1e-20).
 //   controller (one thread, FP64): the accept / reject rule of solver_3d.py:182-208.
 // Field traffic per attempt: 31 complex64 field passes (2+5+2+5+2+5+2+3+4+1).
 #include <math.h>
@@ -65,6 +67,8 @@ struct Ctl3D {
     int cur, done, nonfinite, fail_count, attempts, accepted, rejected, pad_;
     int plasma_groups[8];   // per t-pass slot of the attempt: workgroups deferred to plasma
     int adj_plasma;         // set by the adjoint t-pass when a line's plasma term matters
+    int l2;                 // error norm: 0 max-norm ratio, 1 L2-norm ratio
+    double sumA, sumD;      // L2 norm: sum |A4|^2 and sum |FFT2(Y4 - Y5)|^2 of the attempt
 };
 
 struct Tabs3D {
@@ -950,6 +954,22 @@ __device__ __forceinline__ void block_max_to(unsigned int m, unsigned int *dst, 
     }
 }
 
+// FP64 sum of a per-thread partial over the workgroup, one atomic add per workgroup (the L2
+// error norm).  Called only when ctl->l2 is set, a value uniform over the grid.
+template <int T>
+__device__ __forceinline__ void block_sum_to(float sp, double *dst, double *red, int tid) {
+    double v = (double)sp;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((tid & 63) == 0) red[tid >> 6] = v;
+    __syncthreads();
+    if (tid == 0) {
+        double r = red[0];
+        for (int i = 1; i < T / 64; ++i) r += red[i];
+        atomicAdd(dst, r);
+    }
+}
+
 // MODE 2: X = iFFT2(E (FFT2(B) + h/2 FFT2(Y))), V = E FFT2(B), Q = E (FFT2(B) + h/6 FFT2(Y))
 // MODE 3: X = iFFT2(V + h/2 FFT2(Y)),            Q += h/3 FFT2(Y)
 // MODE 4: X = iFFT2(E (V + h FFT2(Y))),           Q = E (Q + h/3 FFT2(Y))
@@ -963,6 +983,7 @@ __global__ __launch_bounds__(xy_threads(NX, NY)) void xypass_kernel(Fields3D f, 
     using C = XY<NX, NY>;
     __shared__ float2 lds[NX * C::LD];
     __shared__ unsigned int red[C::T / 64 > 0 ? C::T / 64 : 1];
+    __shared__ double reds[C::T / 64 > 0 ? C::T / 64 : 1];
     if (ctl->done) return;
     const int w = blockIdx.x;
     const int tid = threadIdx.x;
@@ -982,12 +1003,16 @@ __global__ __launch_bounds__(xy_threads(NX, NY)) void xypass_kernel(Fields3D f, 
     fft2_lds<NX, NY, 1>(lds, tid, tb);
     if (MODE == 6) {
         unsigned int m = 0;
+        float sq = 0.0f;
 #pragma unroll
         for (int i = 0; i < C::EPT; ++i) {
-            const unsigned int v = __float_as_uint(cabs2(lds[lslot<NY>(tid + C::T * i)]));
+            const float a2 = cabs2(lds[lslot<NY>(tid + C::T * i)]);
+            const unsigned int v = __float_as_uint(a2);
             m = v > m ? v : m;
+            sq += a2;
         }
         block_max_to<C::T>(m, &ctl->maxD_bits, red, tid);
+        if (ctl->l2) block_sum_to<C::T>(sq, &ctl->sumD, reds, tid);
         return;
     }
     if (MODE == 2) {
@@ -1030,15 +1055,19 @@ __global__ __launch_bounds__(xy_threads(NX, NY)) void xypass_kernel(Fields3D f, 
         }
     } else {  // MODE 5
         unsigned int m = 0;
+        float sq = 0.0f;
 #pragma unroll
         for (int i = 0; i < C::EPT; ++i) {
             const int e = tid + C::T * i;
             const float2 a = caxpy(Qp[e], h / 6.0f, lds[lslot<NY>(e)]);
-            const unsigned int v = __float_as_uint(cabs2(a));
+            const float a2 = cabs2(a);
+            const unsigned int v = __float_as_uint(a2);
             m = v > m ? v : m;
+            sq += a2;
             lds[lslot<NY>(e)] = a;
         }
         block_max_to<C::T>(m, &ctl->maxA_bits, red, tid);
+        if (ctl->l2) block_sum_to<C::T>(sq, &ctl->sumA, reds, tid);
     }
     __syncthreads();
     fft2_lds<NX, NY, -1>(lds, tid, tb);
@@ -1182,6 +1211,7 @@ template <int MODE>
 __global__ __launch_bounds__(x128::T) void xy128_kernel(Fields3D f, Tabs3D tb, Ctl3D *ctl) {
     __shared__ float2 lds[x128::LDS];
     __shared__ unsigned int red[x128::T / 64];
+    __shared__ double reds[x128::T / 64];
     if (ctl->done) return;
     const int w = blockIdx.x;
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
@@ -1195,12 +1225,16 @@ __global__ __launch_bounds__(x128::T) void xy128_kernel(Fields3D f, Tabs3D tb, C
     fft2_128(b, Yp, lds, wv, lane, tb.tw128);
     if (MODE == 6) {
         unsigned int m = 0;
+        float sq = 0.0f;
 #pragma unroll
         for (int n = 0; n < 16; ++n) {
-            const unsigned int u = __float_as_uint(cabs2(b[n]));
+            const float a2 = cabs2(b[n]);
+            const unsigned int u = __float_as_uint(a2);
             m = u > m ? u : m;
+            sq += a2;
         }
         block_max_to<x128::T>(m, &ctl->maxD_bits, red, tid);
+        if (ctl->l2) block_sum_to<x128::T>(sq, &ctl->sumD, reds, tid);
         return;
     }
     float2 a[16];
@@ -1236,13 +1270,17 @@ __global__ __launch_bounds__(x128::T) void xy128_kernel(Fields3D f, Tabs3D tb, C
         }
     } else {  // MODE 5
         unsigned int m = 0;
+        float sq = 0.0f;
 #pragma unroll
         for (int n = 0; n < 16; ++n) {
             a[n] = caxpy(Qp[n * x128::T + tid], h / 6.0f, b[n]);
-            const unsigned int u = __float_as_uint(cabs2(a[n]));
+            const float a2 = cabs2(a[n]);
+            const unsigned int u = __float_as_uint(a2);
             m = u > m ? u : m;
+            sq += a2;
         }
         block_max_to<x128::T>(m, &ctl->maxA_bits, red, tid);
+        if (ctl->l2) block_sum_to<x128::T>(sq, &ctl->sumA, reds, tid);
     }
     ifft2_128(a, Xp, lds, wv, lane, tb.tw128);
 }
@@ -1668,6 +1706,7 @@ __global__ __launch_bounds__(1024) void row_kernel(Fields3D f, Adj3D a, Tabs3D t
     constexpr bool ADJ = MODE >= 20;
     __shared__ float2 lines[16 * LP];
     __shared__ unsigned int red[16];
+    __shared__ double reds[16];
     if (!ADJ && ctl->done) return;
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int rpp = nx / 16;
@@ -1704,6 +1743,7 @@ __global__ __launch_bounds__(1024) void row_kernel(Fields3D f, Adj3D a, Tabs3D t
     const float2 *in = (MODE == 5) ? f.X : (ADJ ? a.Xb : f.Yw);
     fwd(in, b);
     unsigned int m = 0;
+    float sq = 0.0f;
 #pragma unroll
     for (int r = 0; r < EPT; ++r) {
         const size_t e = rb + lane + 64 * r;
@@ -1716,11 +1756,15 @@ __global__ __launch_bounds__(1024) void row_kernel(Fields3D f, Adj3D a, Tabs3D t
             f.Q[e] = cmul(E, caxpy(f.Q[e], h / 3.0f, b[r]));
         } else if (MODE == 5) {
             o[r] = caxpy(f.Q[e], h / 6.0f, b[r]);
-            const unsigned int u = __float_as_uint(cabs2(o[r]));
+            const float a2 = cabs2(o[r]);
+            const unsigned int u = __float_as_uint(a2);
             m = u > m ? u : m;
+            sq += a2;
         } else if (MODE == 6) {
-            const unsigned int u = __float_as_uint(cabs2(b[r]));
+            const float a2 = cabs2(b[r]);
+            const unsigned int u = __float_as_uint(a2);
             m = u > m ? u : m;
+            sq += a2;
         } else if (MODE == 25) {
             a.Qb[e] = cscale(b[r], inv);
         } else if (MODE == 24) {
@@ -1757,7 +1801,10 @@ __global__ __launch_bounds__(1024) void row_kernel(Fields3D f, Adj3D a, Tabs3D t
         }
         inv_store(ab, a.Blin, 1.0f);
     }
-    if (MODE == 5 || MODE == 6) block_max_to<1024>(m, MODE == 5 ? &ctl->maxA_bits : &ctl->maxD_bits, red, tid);
+    if (MODE == 5 || MODE == 6) {
+        block_max_to<1024>(m, MODE == 5 ? &ctl->maxA_bits : &ctl->maxD_bits, red, tid);
+        if (ctl->l2) block_sum_to<1024>(sq, MODE == 5 ? &ctl->sumA : &ctl->sumD, reds, tid);
+    }
     if (MODE == 2 || MODE == 3 || MODE == 4 || MODE == 5) inv_store(o, f.X, 1.0f / (float)NY);
     if (MODE == 22 || MODE == 23 || MODE == 24) inv_store(o, a.Yb, 1.0f);
 }
@@ -1781,17 +1828,23 @@ __device__ void controller(Ctl3D *c) {
     if (c->done) return;
     const double h = c->h;
     const unsigned int ab = c->maxA_bits, db = c->maxD_bits;
+    const double sa = c->sumA, sd = c->sumD;
     c->maxA_bits = 0;
     c->maxD_bits = 0;
+    c->sumA = 0.0;
+    c->sumD = 0.0;
     c->attempts += 1;
     // An overflowed norm (FP32 range; the FP64 reference would see a huge finite error)
     // counts as err = +inf: the attempt is rejected and h falls to min_step, the value the
     // reference's factor (tol/err)^0.2 gives for any err that large.  Accepting such an
     // attempt (forced min-step or replayed accept) stops the run as non-finite.
-    const bool bad = ab >= 0x7f800000u || db >= 0x7f800000u;
+    // L2 (solver_3d.py:252-254 of the MPC runner's solver): the same ratio over sums
+    const bool bad = ab >= 0x7f800000u || db >= 0x7f800000u ||
+                     (c->l2 && !(isfinite(sa) && isfinite(sd)));
+    const double num = c->l2 ? sqrt(sd) : sqrt((double)__uint_as_float(db));
+    const double den = c->l2 ? sqrt(sa) : sqrt((double)__uint_as_float(ab));
     const double err = bad ? __longlong_as_double(0x7ff0000000000000ll)
-                           : (h / 10.0) * sqrt((double)__uint_as_float(db)) /
-                                 (sqrt((double)__uint_as_float(ab)) + 1e-20);
+                           : (h / 10.0) * num / (den + 1e-20);
     c->err = err;
     const bool forced = h <= c->min_step * 1.01;
     const bool acc = c->replay ? (c->replay_acc != 0) : (err < c->tol || forced);
@@ -1850,6 +1903,8 @@ __global__ void begin_kernel(Ctl3D *c, double L, double tol, double max_step, do
     c->replay_acc = 0;
     c->maxA_bits = 0;
     c->maxD_bits = 0;
+    c->sumA = 0.0;
+    c->sumD = 0.0;
     c->done = 0;
     c->nonfinite = 0;
     c->fail_count = 0;
@@ -2696,6 +2751,15 @@ int uppe3d_adjoint_step(uppe3d_plan *p, double h, const void *B, const void *X2,
     RET(tadj(p, (const float2 *)X2, a.Yb, a.Xb, nullptr, h, s));
     RET(xadj(p, 2, a, hf, s));
     RET(tadj(p, (const float2 *)B, a.Yb, a.Bb, a.Blin, h, s));
+    return UPPE_OK;
+}
+
+int uppe3d_set_error_norm(uppe3d_plan *p, int norm) {
+    UPPE_CHECK(p, "uppe3d_set_error_norm: null plan");
+    UPPE_CHECK(norm == 0 || norm == 1, "uppe3d_set_error_norm: norm must be 0 (L-inf) or 1 (L2)");
+    UPPE_HIP(hipDeviceSynchronize());
+    UPPE_HIP(hipMemcpy((char *)p->ctl + offsetof(Ctl3D, l2), &norm, sizeof norm,
+                       hipMemcpyHostToDevice));
     return UPPE_OK;
 }
 

@@ -1,0 +1,275 @@
+"""The 3+1D MPC cavity runner (``src/3+1D-ERK4(3)IP UPPE/sim.py:11-352``) on the device.
+
+One cavity cycle (``sim.py:319-334``): the forward segment sequence air / fused silica x 4 /
+air, the curved mirror of radius R, the same segments in reverse order, and the flat-mirror
+loss sqrt(T_per_round); a snapshot after cycle 1 and every even cycle (``:350-351``).
+
+The field never leaves the GPU.  Each segment runs on the air or fused-silica solver
+(``project_amd.mpc_solver_3d``, one HIP plan each) and stays in that plan's state; the mirror
+phase, the loss factor and every snapshot sum run on that state (``uppe3d_apply_mirror``,
+``uppe3d_diagnostics``), and the next segment loads the solver's output field directly.  The
+host sees only the per-cycle diagnostics.
+
+Interface kept: ``MPC_Final_Runner()`` with the reference's physical constants
+(``sim.py:12-75``; any of them can be overridden by keyword), ``calculate_eigenmode``,
+``generate_initial_field``, ``propagate_segment``, ``apply_curved_mirror``, ``save_snapshot``
+(the ``cycle_XX_<label>_DIAG.npz`` keys of ``:245-249`` and, with ``plot=True``, the six-panel
+figure of ``:251-302``) and ``run``.  Fields are normalised by the runner's ``A_scale``
+(``:98-118``) as in the reference, as torch complex64 tensors on the GPU.
+Differences: N_t must be a power of two from 128 to 2048 (default 2048; the reference uses
+2046, which the HIP t transforms do not take); ``run`` returns the per-cycle log.
+"""
+import os
+import time
+
+import numpy as np
+
+from . import diagnostics as _dg
+from .mpc_solver_3d import ERK43IP_UPPE_3D_Optimized
+
+C0 = 299792458.0
+
+
+class MPC_Final_Runner:
+    DEFAULTS = dict(
+        output_dir="mpc_results_fixed_zoom_1",
+        wavelength=1064e-9, pulse_duration=10.7e-12, avg_power=50.0, rep_rate=500e3,
+        L_cavity=212.5e-3, R_concave=300e-3,
+        d_fs=12e-3, d_air_1=30e-3, d_air_mid=25e-3, n_plates=4,
+        cycles=33, total_trans=53.9 / 100.0,
+        window_width=2.0e-3, Nxy=256, Nt=2048, time_window=80e-12,
+        n2_fs=3.0e-20, plot=False, verbose=True,
+        viz_wl_min_nm=1050.0, viz_wl_max_nm=1080.0, viz_time_ps_min=-30.0,
+        viz_time_ps_max=30.0)
+
+    def __init__(self, **overrides):
+        unknown = set(overrides) - set(self.DEFAULTS)
+        if unknown:
+            raise TypeError("unknown MPC_Final_Runner parameters: %s" % sorted(unknown))
+        cfg = dict(self.DEFAULTS, **overrides)
+        for k, v in cfg.items():
+            setattr(self, k, v)
+        if self.output_dir and not os.path.exists(self.output_dir):
+            os.makedirs(self.output_dir)
+        # sim.py:17-25
+        self.pulse_energy = self.avg_power / self.rep_rate
+        self.fwhm_to_tau = 1.0 / (2 * np.sqrt(np.log(2)))
+        self.tau = self.pulse_duration * self.fwhm_to_tau
+        # sim.py:27-43: cavity geometry; the last air gap fills the cavity length
+        self.w0_waist = self.calculate_eigenmode(self.L_cavity, self.R_concave, self.wavelength)
+        self._log("[Physics] Calculated Eigenmode Waist w0 = %.2f um" % (self.w0_waist * 1e6))
+        occupied = self.d_air_1 + self.n_plates * self.d_fs + (self.n_plates - 1) * self.d_air_mid
+        self.d_air_last = self.L_cavity - occupied
+        if self.d_air_last <= 0:
+            raise ValueError("segments longer than the cavity (last air gap %g m)" % self.d_air_last)
+        self.T_per_round = self.total_trans ** (1 / self.cycles)
+        # sim.py:45-68: grid and the two solvers
+        w, n, tw = self.window_width, self.Nxy, self.time_window
+        x = np.linspace(-w / 2, w / 2, n)
+        y = np.linspace(-w / 2, w / 2, n)
+        t = np.linspace(-tw / 2, tw / 2, self.Nt)
+        self._log("[Init] Allocating Solvers (Grid: %dx%dx%d, Window: %gmm)..."
+                  % (n, n, self.Nt, w * 1e3))
+        self.solver_fs = ERK43IP_UPPE_3D_Optimized(
+            x, y, t, material="fused_silica", n2=self.n2_fs, center_wavelength=self.wavelength,
+            use_shock=False, use_raman=False, precision="single")
+        self.solver_air = ERK43IP_UPPE_3D_Optimized(
+            x, y, t, material="air", n2=0.0, center_wavelength=self.wavelength,
+            use_shock=False, use_raman=False, precision="single")
+        self.A_scale = 1.0
+        # (solver, tensor, tensor version): the solver whose state is that field (an in-place
+        # change of the tensor bumps its version, and the field is then loaded again)
+        self._holder = None
+        self.history = []
+
+    def _log(self, msg):
+        if self.verbose:
+            print(msg)
+
+    # -- sim.py:77-80 ----------------------------------------------------------------------
+    def calculate_eigenmode(self, L, R, wl):
+        if L >= R:
+            raise ValueError("Unstable Cavity!")
+        return np.sqrt((wl / np.pi) * np.sqrt(L * (R - L)))
+
+    def segments(self):
+        """Forward sequence of sim.py:308-316: air, (fs, air) x (n - 1), fs, air."""
+        seq = [(self.d_air_1, "air")]
+        for i in range(self.n_plates):
+            seq.append((self.d_fs, "fs"))
+            seq.append((self.d_air_mid if i < self.n_plates - 1 else self.d_air_last, "air"))
+        return seq
+
+    # -- sim.py:98-118 ---------------------------------------------------------------------
+    def generate_initial_field(self):
+        import torch
+        s = self.solver_fs
+        # float32 grids and field (the solver's grids are float32 in the reference), FP64 energy
+        X, Y = np.meshgrid(s.x.astype(np.float32), s.y.astype(np.float32), indexing="ij")
+        T = s.t.astype(np.float32)
+        spatial = np.exp(-(X ** 2 + Y ** 2) / np.float32(self.w0_waist ** 2))
+        temporal = np.exp(-(T ** 2) / np.float32(self.tau ** 2))
+        field = spatial[:, :, None] * temporal[None, None, :]
+        E = float(np.sum(np.abs(field).astype(np.float64) ** 2) * s.dx * s.dy * s.dt)
+        phys = field * np.float32(np.sqrt(self.pulse_energy / E))
+        self.A_scale = float(np.max(np.abs(phys)))
+        if self.A_scale < 1e-30:
+            self.A_scale = 1.0
+        a = torch.from_numpy((phys / np.float32(self.A_scale)).astype(np.complex64)).to(s.device)
+        self._holder = None
+        return a
+
+    # -- solver plumbing -------------------------------------------------------------------
+    def _solver(self, medium_type):
+        return self.solver_air if medium_type == "air" else self.solver_fs
+
+    def _held(self, field_norm):
+        """The solver whose state is ``field_norm`` (loading it into the fused-silica solver
+        if no solver holds it)."""
+        h = self._holder
+        if h is not None and h[1] is field_norm and field_norm._version == h[2]:
+            return h[0]
+        s = self.solver_fs
+        s.load(self._clean(field_norm) * self.A_scale)
+        self._holder = (s, field_norm, field_norm._version)
+        return s
+
+    def _out(self, solver):
+        a = solver.field()
+        a.mul_(1.0 / self.A_scale)
+        self._holder = (solver, a, a._version)
+        return a
+
+    @staticmethod
+    def _clean(field_norm):
+        import torch
+        if bool(torch.isnan(field_norm).any()):
+            return torch.nan_to_num(field_norm)
+        return field_norm
+
+    # -- sim.py:120-137 --------------------------------------------------------------------
+    def propagate_segment(self, field_norm, distance, medium_type):
+        s = self._solver(medium_type)
+        if medium_type == "air":
+            kw = dict(max_step=min(distance, 5e-3), tol=1e-4)
+        else:
+            kw = dict(max_step=20e-6, tol=1e-5)
+        s.advance(self._clean(field_norm) * self.A_scale, distance, **kw)
+        return self._out(s)
+
+    # -- sim.py:82-96 ----------------------------------------------------------------------
+    def apply_curved_mirror(self, field_norm):
+        s = self._held(field_norm)
+        _dg.apply_mirror(s, self.R_concave)
+        return self._out(s)
+
+    def apply_loss(self, field_norm):
+        """A_norm *= sqrt(T_per_round) (sim.py:334), on the holding solver's state."""
+        s = self._held(field_norm)
+        _dg.apply_mirror(s, 0.0, float(np.sqrt(self.T_per_round)))
+        return self._out(s)
+
+    # -- sim.py:187-302 --------------------------------------------------------------------
+    def save_snapshot(self, cycle, field_norm, label):
+        s = self._held(field_norm)
+        snap = _dg.snapshot(s, self.w0_waist)
+        if self.output_dir:
+            _dg.save_snapshot_npz(
+                os.path.join(self.output_dir, "cycle_%02d_%s_DIAG.npz" % (cycle, label)),
+                snap, cycle, label)
+            if self.plot:
+                self._plot(cycle, label, snap)
+        return snap
+
+    def _plot(self, cycle, label, d):
+        import matplotlib
+        matplotlib.use("Agg")
+        import matplotlib.pyplot as plt
+
+        def norm(a):
+            a = np.asarray(a)
+            m = np.max(a) if np.max(a) > 0 else 1.0
+            return a / (m + 1e-30)
+
+        wl, t_ps = d["wl_nm"], d["t_s"] * 1e12
+        valid = d["freq_abs_Hz"] > 1e9
+        mw = (wl > self.viz_wl_min_nm) & (wl < self.viz_wl_max_nm) & valid
+        mt = (t_ps >= self.viz_time_ps_min) & (t_ps <= self.viz_time_ps_max)
+        ext = [-self.window_width * 500, self.window_width * 500] * 2
+        fig = plt.figure(figsize=(18, 10), constrained_layout=True)
+        gs = fig.add_gridspec(2, 3)
+        ax = fig.add_subplot(gs[0, 0])
+        for k, lab, lw in (("spec_smf", "SMF", 1.8), ("spec_full_incoh", "Full Beam", 1.0),
+                           ("spec_center", "Center", 1.0)):
+            ax.plot(wl[mw], norm(d[k])[mw], lw=lw, label=lab)
+        ax.set_xlim(self.viz_wl_min_nm, self.viz_wl_max_nm)
+        ax.set_title("Spectrum - %s Cycle %d" % (label, cycle))
+        ax.legend()
+        ax = fig.add_subplot(gs[0, 1])
+        ax.plot(t_ps[mt], norm(d["I_center_t"])[mt], label="Center")
+        ax.plot(t_ps[mt], norm(d["I_smf_t"])[mt], label="SMF")
+        ax.set_xlim(self.viz_time_ps_min, self.viz_time_ps_max)
+        ax.set_title("Temporal - Peak@t=%.2fps\nFWHM(c)=%.2fps, FWHM(S)=%.2fps"
+                     % (t_ps[d["it_peak"]], d["fwhm_center"] * 1e12, d["fwhm_smf"] * 1e12))
+        ax.legend()
+        ax = fig.add_subplot(gs[0, 2])
+        ax.plot(t_ps[mt], norm(d["P_t"])[mt], color="k")
+        ax.set_xlim(self.viz_time_ps_min, self.viz_time_ps_max)
+        ax.set_title("Total Power vs Time")
+        amp = np.abs(d["A_xy"])
+        for pos, img, cmap, title in (
+                (gs[1, 0], d["fluence"], "inferno", "Near-field Fluence\nE=%.3eJ, wrms=%.1fum"
+                 % (d["energy_total"], d["wx_rms_m"] * 1e6)),
+                (gs[1, 1], amp, "viridis", "Amplitude |A| at t_peak=%.2fps" % t_ps[d["it_peak"]]),
+                (gs[1, 2], np.where(amp >= 0.10 * (np.max(amp) + 1e-30), np.angle(d["A_xy"]),
+                                    np.nan), "twilight", "Phase arg(A)")):
+            ax = fig.add_subplot(pos)
+            kw = dict(vmin=-np.pi, vmax=np.pi) if cmap == "twilight" else {}
+            im = ax.imshow(img, cmap=cmap, origin="lower", extent=ext, **kw)
+            ax.set_title(title)
+            fig.colorbar(im, ax=ax)
+        plt.savefig(os.path.join(self.output_dir, "cycle_%02d_%s_MULTI_DIAG.png" % (cycle, label)),
+                    dpi=140)
+        plt.close(fig)
+
+    # -- sim.py:304-352 --------------------------------------------------------------------
+    def cycle(self, A_norm):
+        """One round trip: forward segments, curved mirror, backward segments, loss."""
+        seq = self.segments()
+        for dist, mat in seq:
+            A_norm = self.propagate_segment(A_norm, dist, mat)
+        A_norm = self.apply_curved_mirror(A_norm)
+        for dist, mat in seq[::-1]:
+            A_norm = self.propagate_segment(A_norm, dist, mat)
+        return self.apply_loss(A_norm)
+
+    def run(self, cycles=None, snapshots=True):
+        cycles = self.cycles if cycles is None else cycles
+        A_norm = self.generate_initial_field()
+        self._log("\n[Run] MPC simulation, %d cycles..." % cycles)
+        if snapshots:
+            self.save_snapshot(0, A_norm, "Init")
+        self.history = []
+        for i in range(1, cycles + 1):
+            t0 = time.time()
+            A_norm = self.cycle(A_norm)
+            s = self._held(A_norm)
+            snap = _dg.snapshot(s, self.w0_waist)
+            elapsed = time.time() - t0
+            flu = snap["fluence"]
+            has_nan = bool(np.isnan(flu).any() or not np.isfinite(snap["energy_total"]))
+            max_flu = float(np.nanmax(flu)) if flu.size else 0.0
+            self._log("Cycle %-2d | Time: %.1fs | MaxFluence: %.2e J/m^2 | Status: %s"
+                      % (i, elapsed, max_flu, "WARNING: NaN" if has_nan else "OK"))
+            self.history.append(dict(cycle=i, seconds=elapsed, max_fluence=max_flu,
+                                     energy=snap["energy_total"], nan=has_nan))
+            if has_nan:
+                self._log("!!! Simulation Stopped due to NaN !!!")
+                break
+            if snapshots and (i == 1 or i % 2 == 0) and self.output_dir:
+                _dg.save_snapshot_npz(
+                    os.path.join(self.output_dir, "cycle_%02d_Flat_DIAG.npz" % i), snap, i, "Flat")
+                if self.plot:
+                    self._plot(i, "Flat", snap)
+        self.field = A_norm
+        return self.history

@@ -29,6 +29,11 @@ def _fftfreq(n, d):
 
 
 class ERK43IP_UPPE_3D_Optimized:
+    # embedded error norm: "linf" (solver_3d.py:151-153) or "l2"; valid_mask False drops the
+    # negative-absolute-frequency mask from D and N (both switched by the MPC solver variant)
+    error_norm = "linf"
+    valid_mask = True
+
     def __init__(self, x_grid, y_grid, t_grid, medium, precision="single"):
         import torch
 
@@ -65,6 +70,8 @@ class ERK43IP_UPPE_3D_Optimized:
         self.anti_alias_filter = np.exp(-(f / (0.85 * fmax)) ** 16)
         self.omega_abs = omega + self.omega0
         self.valid_freq_mask = (self.omega_abs > 0).astype(np.float64)
+        if not self.valid_mask:
+            self.valid_freq_mask = np.ones_like(self.valid_freq_mask)
         self.A_scale = 1.0
         self.device = torch.device("cuda")
         self._plan = None
@@ -86,6 +93,8 @@ class ERK43IP_UPPE_3D_Optimized:
         self.k0, self.k1 = k0, k1
         self.kref_w = k0 + k1 * self.omega
         self.valid_w = ((self.omega + w0) > 0).astype(np.float64)
+        if not self.valid_mask:
+            self.valid_w = np.ones_like(self.valid_w)
 
     def dispersion_operator(self):
         """The full D(x, y, omega) in FP64 (host; for inspection and tests)."""
@@ -139,6 +148,9 @@ class ERK43IP_UPPE_3D_Optimized:
             ctypes.byref(d), _lib.ptr(self._ws), nbytes, *[_lib.host_ptr(a) for a in arrs],
             _lib.stream_handle(), ctypes.byref(plan)), "uppe3d_plan_create")
         self._plan = plan
+        if self.error_norm == "l2":
+            _lib.check(lib.uppe3d_set_error_norm(plan, _lib.UPPE_NORM_L2),
+                       "uppe3d_set_error_norm")
         self._desc = d
 
     def __del__(self):
@@ -178,6 +190,13 @@ class ERK43IP_UPPE_3D_Optimized:
 
     def propagate(self, A0, L, tol=1e-4, max_step=1e-2, min_step=1e-9, return_gpu=False,
                   check_every=8, max_attempts=0):
+        self.advance(A0, L, tol, max_step, min_step, check_every, max_attempts)
+        return self._finish(return_gpu)
+
+    def advance(self, A0, L, tol=1e-4, max_step=1e-2, min_step=1e-9, check_every=8,
+                max_attempts=0):
+        """propagate without the output spectrum: the field stays in the plan's state (read it
+        with ``field()``, or hand the plan to the diagnostics).  Returns z."""
         lib = _lib.load()
         self.load(A0)
         s = _lib.stream_handle()
@@ -193,7 +212,7 @@ class ERK43IP_UPPE_3D_Optimized:
             st = self.status()
             print("\n[Solver] Interrupted at z=%.3f mm" % (st.z * 1000))
         self.last_status = self.status()
-        return self._finish(return_gpu)
+        return float(self.last_status.z)
 
     def propagate_replay(self, A0, steps, return_gpu=False):
         """Replays a step sequence [(h, err, accepted), ...] (e.g. an oracle log)."""

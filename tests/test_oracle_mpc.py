@@ -1,0 +1,138 @@
+"""The MPC-solver oracle (oracle/mpc.py) pinned against what fixes it independently:
+published refractive indices (Malitson 1965 for fused silica, Malitson & Dodge 1972 for the
+ordinary ray of sapphire), the pure-diffraction operator of a constant-index medium, the
+scalar ERK4(3) recursion on a CW plane wave (where the L2 and L-infinity norms coincide), and
+the eigenmode of the half-symmetric cavity (a linear all-air round trip returns the beam)."""
+import numpy as np
+import pytest
+
+from oracle import diagnostics as od
+from oracle import mpc
+from oracle.uppe3d import C_LIGHT
+from project_amd import mpc as pmpc
+from project_amd import mpc_solver_3d as ms
+
+
+def index_at(m, lam_um):
+    return float(m.refractive_index(np.array([2 * np.pi * C_LIGHT / (lam_um * 1e-6)]))[0])
+
+
+@pytest.mark.parametrize("lam,n", [(0.4047, 1.46961), (0.5461, 1.46008), (0.6328, 1.45702),
+                                   (1.0642, 1.44963), (1.5296, 1.44427)])
+def test_fused_silica_index_published(lam, n):
+    assert index_at(mpc.SellmeierKerr("fused_silica"), lam) == pytest.approx(n, abs=1e-5)
+
+
+@pytest.mark.parametrize("lam,n", [(0.6328, 1.7659), (1.064, 1.7545)])
+def test_sapphire_ordinary_index_published(lam, n):
+    assert index_at(mpc.SellmeierKerr("sapphire"), lam) == pytest.approx(n, abs=1e-4)
+
+
+def test_index_clip_guard_air_and_fallback():
+    m = mpc.SellmeierKerr("fused_silica")
+    # outside 0.2-6 um the wavelength is clipped; omega = 0 is guarded
+    assert index_at(m, 20.0) == index_at(m, 6.0)
+    assert index_at(m, 0.1) == index_at(m, 0.2)
+    assert np.isfinite(m.refractive_index(np.array([0.0]))[0])
+    assert np.all(mpc.SellmeierKerr("air").refractive_index(np.linspace(1e14, 3e15, 5)) == 1.00027)
+    assert index_at(mpc.SellmeierKerr("unobtainium"), 1.0) == index_at(m, 1.0)
+
+
+def test_product_material_table_matches_oracle():
+    w = 2 * np.pi * C_LIGHT / np.linspace(0.15e-6, 8e-6, 97)
+    for name in ("fused_silica", "sapphire", "air", "bk7"):
+        a = ms.BuiltinMaterial(name, 3e-20, 1064e-9).refractive_index(w)
+        b = mpc.SellmeierKerr(name, 3e-20, 1064e-9).refractive_index(w)
+        np.testing.assert_allclose(a, b, rtol=1e-15)
+
+
+def grid(n=32, nt=64, hw=1e-3, tw=80e-12):
+    x = np.linspace(-hw, hw, n)
+    t = np.linspace(-tw / 2, tw / 2, nt)
+    return x, x.copy(), t
+
+
+def test_air_operator_is_pure_diffraction():
+    x, y, t = grid()
+    o = mpc.OracleMPC3D(x, y, t, mpc.SellmeierKerr("air", 0.0, 1064e-9, use_shock=False))
+    k = 1.00027 * (o.omega + o.m.omega0) / C_LIGHT
+    kz = np.sqrt((k[None, None, :] ** 2 - o.kperp2[:, :, None]).astype(complex))
+    np.testing.assert_allclose(o.D, 1j * (kz - k[None, None, :]), rtol=1e-8, atol=1e-9)
+    assert np.all(o.mask == 1.0)
+
+
+def test_l2_norm_on_cw_plane_wave_equals_scalar_recursion():
+    """A constant field stays in the (0, 0, 0) mode under Kerr SPM; D(0, 0, 0) = 0.  Then
+    both norms equal |A4 - A3| / |A4| of the scalar ERK4(3) recursion of dA/dz = i g |A|^2 A."""
+    x, y, t = grid(16, 32, hw=1e-3)
+    m = mpc.SellmeierKerr("fused_silica", 3e-20, 1064e-9, use_shock=False)
+    A = np.full((16, 16, 32), 0.7 + 0.2j)
+    errs = {}
+    for norm in ("l2", "linf"):
+        o = mpc.OracleMPC3D(x, y, t, m)
+        o.err_norm = norm
+        o.damp[:] = 0.0
+        o.A_scale = 3e8
+        h = 2e-3
+        A4t, errs[norm] = o.step(A, h)
+    assert errs["l2"] == pytest.approx(errs["linf"], rel=1e-9)
+    # scalar recursion (normalised field a, g = (omega0 / c) n2 A_scale^2)
+    g = m.omega0 / C_LIGHT * m.n2 * 3e8 ** 2
+    f = lambda a: 1j * g * abs(a) ** 2 * a
+    a = 0.7 + 0.2j
+    k1 = f(a)
+    k2 = f(a + h / 2 * k1)
+    k3 = f(a + h / 2 * k2)
+    k4 = f(a + h * k3)
+    a4 = a + h / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+    k5 = f(a4)
+    a3 = a + h / 6 * (k1 + 2 * k2 + 2 * k3) + h / 30 * (2 * k4 + 3 * k5)
+    assert errs["l2"] == pytest.approx(abs(a4 - a3) / abs(a4), rel=1e-6)
+    assert errs["l2"] > 0
+    np.testing.assert_allclose(A4t, a4, rtol=1e-9)
+
+
+def test_l2_differs_from_linf_on_a_beam():
+    x, y, t = grid(16, 64)
+    m = mpc.SellmeierKerr("fused_silica", 3e-20, 1064e-9, use_shock=False)
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    A = np.exp(-(X ** 2 + Y ** 2) / (2e-4) ** 2)[:, :, None] * np.exp(-t ** 2 / 6e-12 ** 2)
+    e = {}
+    for norm in ("l2", "linf"):
+        o = mpc.OracleMPC3D(x, y, t, m)
+        o.err_norm = norm
+        o.A_scale = 5e8
+        e[norm] = o.step(A.astype(complex), 1e-4)[1]
+    assert e["l2"] > 0 and e["linf"] > 0 and abs(e["l2"] / e["linf"] - 1) > 1e-3
+
+
+def test_linear_air_round_trip_returns_the_eigenmode():
+    """Flat mirror - air gap L - curved mirror R - air gap L: the Gaussian of waist
+    w0^2 = (lambda / pi) sqrt(L (R - L)) (the product's calculate_eigenmode) comes back."""
+    lam, L, R = 1064e-9, 212.5e-3, 300e-3
+    w0 = pmpc.MPC_Final_Runner.calculate_eigenmode(None, L, R, lam)
+    assert w0 == pytest.approx(np.sqrt(lam / np.pi * np.sqrt(L * (R - L))), rel=1e-15)
+    x, y, t = grid(64, 64, hw=1e-3, tw=80e-12)
+    air = mpc.OracleMPC3D(x, y, t, mpc.SellmeierKerr("air", 0.0, lam, use_shock=False))
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    A = np.exp(-(X ** 2 + Y ** 2) / w0 ** 2)[:, :, None] * np.exp(-t ** 2 / (6e-12) ** 2)
+    out, logs = mpc.cavity_cycle({"air": air}, A.astype(complex), 1.0, [(L, "air")], R, 0.81,
+                                 air.omega, air.m.omega0, x, y)
+    # |A| returns (times sqrt(T)); the Gouy phase is a constant
+    err = np.linalg.norm(np.abs(out) - 0.9 * np.abs(A)) / np.linalg.norm(0.9 * np.abs(A))
+    assert err < 2e-3, err
+    # a wrong mirror sign (diverging mirror) does not return the beam
+    bad = mpc.propagate_segment(air, A.astype(complex), 1.0, L, "air")[0]
+    bad = od.mirror(bad, x, y, air.omega, air.m.omega0, -R)
+    bad = mpc.propagate_segment(air, bad, 1.0, L, "air")[0]
+    assert np.linalg.norm(np.abs(bad) - np.abs(A)) / np.linalg.norm(np.abs(A)) > 0.2
+
+
+def test_runner_segments_fill_the_cavity():
+    r = pmpc.MPC_Final_Runner.__new__(pmpc.MPC_Final_Runner)
+    r.d_air_1, r.d_fs, r.d_air_mid, r.n_plates = 30e-3, 12e-3, 25e-3, 4
+    r.d_air_last = 212.5e-3 - (30e-3 + 4 * 12e-3 + 3 * 25e-3)
+    seq = r.segments()
+    assert [m for _, m in seq] == ["air", "fs", "air", "fs", "air", "fs", "air", "fs", "air"]
+    assert sum(d for d, _ in seq) == pytest.approx(212.5e-3, rel=1e-15)
+    assert seq[-1][0] == pytest.approx(59.5e-3)
