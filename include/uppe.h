@@ -157,6 +157,45 @@ int uppe3d_adjoint_step(uppe3d_plan *p, double h, const void *B, const void *X2,
 int uppe3d_adjoint_check(uppe3d_plan *p, void *stream, int *plasma_lines);
 
 /* ------------------------------------------------------------------------------------------ */
+/* Real-field (carrier-resolved) 3+1D solver                                                  */
+/* ------------------------------------------------------------------------------------------ */
+/* Replaces the ERK4(3)-IP step of src/3+1D-UPPE-ERK4(3)IP-real-field/solver_3d.py:113-172
+ * (erk43_step, with its low-order error estimate) and the response of media_physics.py:11-118
+ * (the plasma / Drude-current kernel) and :258-318 (calc_nonlinear_response).  The field is
+ * real float32 [nx][ny][nt]; spectra are complex64 [nx][ny][nt/2+1].  nx, ny: powers of two in
+ * 16..256; nt: a power of two in 64..2048.  The step controller stays with the caller
+ * (solver_3d.py:174-241): uppe_rf3d_attempt returns err and leaves the candidate field aside,
+ * uppe_rf3d_accept makes it the state. */
+typedef struct {
+    int nx, ny, nt, lut_n;
+    double dt;          /* s */
+    double i_coef;      /* I = i_coef E^2 (0.5 eps0 c n0) */
+    double chi3_coef;   /* eps0 chi3 */
+    double chi5_coef;   /* eps0 chi5 (0: no fifth-order term) */
+    double av_coef;     /* avalanche rate = av_coef I (0: none) */
+    double rate_rec;    /* 1 / tau_rec */
+    double rho_nt, Ui, e_charge, m_e, tau_c;   /* SI; rounded to float32 like the reference */
+} uppe_rf3d_desc;
+typedef struct uppe_rf3d_plan uppe_rf3d_plan;
+int uppe_rf3d_workspace_size(const uppe_rf3d_desc *d, size_t *bytes);
+/* D, Pc, Jc: host complex64 [nx][ny][nt/2+1] (solver_3d.py:54-100: the propagator exponent
+ * and the P / J coupling coefficients); filt: float32 [nt/2+1]; damp: float32 [nx][ny];
+ * lut_x, lut_w: the float32 log10(I) grid and PPT rates (media_physics.py:167-183). */
+int uppe_rf3d_plan_create(const uppe_rf3d_desc *d, void *workspace, size_t workspace_bytes,
+                          const void *D, const void *Pc, const void *Jc, const float *filt,
+                          const float *damp, const float *lut_x, const float *lut_w,
+                          void *stream, uppe_rf3d_plan **out);
+int uppe_rf3d_plan_destroy(uppe_rf3d_plan *p);
+/* state = e0 / e_scale (float32, device e0 [nx][ny][nt]) */
+int uppe_rf3d_load(uppe_rf3d_plan *p, const void *e0, double e_scale, void *stream);
+/* one attempt of step h from the state; err = max|E4 - E_low| / (max|E4| + 1e-20) (synchronises) */
+int uppe_rf3d_attempt(uppe_rf3d_plan *p, double h, void *stream, double *err);
+int uppe_rf3d_accept(uppe_rf3d_plan *p);
+/* state * e_scale -> out (real [nx][ny][nt]); rFFT_t of it -> out (complex [nx][ny][nt/2+1]) */
+int uppe_rf3d_store_field(uppe_rf3d_plan *p, void *out, void *stream);
+int uppe_rf3d_store_spectrum(uppe_rf3d_plan *p, void *out, void *stream);
+
+/* ------------------------------------------------------------------------------------------ */
 /* 1D envelope solver (single instance and batched, one workgroup per instance)               */
 /* ------------------------------------------------------------------------------------------ */
 typedef struct {

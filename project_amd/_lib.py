@@ -30,6 +30,13 @@ class Uppe3DStatus(ctypes.Structure):
                 ("nonfinite", c_int), ("fail_count", c_int), ("cur", c_int)]
 
 
+class UppeRf3DDesc(ctypes.Structure):
+    _fields_ = [("nx", c_int), ("ny", c_int), ("nt", c_int), ("lut_n", c_int), ("dt", c_double),
+                ("i_coef", c_double), ("chi3_coef", c_double), ("chi5_coef", c_double),
+                ("av_coef", c_double), ("rate_rec", c_double), ("rho_nt", c_double),
+                ("Ui", c_double), ("e_charge", c_double), ("m_e", c_double), ("tau_c", c_double)]
+
+
 class Uppe1DDesc(ctypes.Structure):
     _fields_ = [("nt", c_int), ("dt", c_double), ("omega0", c_double), ("gamma", c_double),
                 ("use_ss", c_int), ("use_raman", c_int), ("f_R", c_double),
@@ -89,6 +96,15 @@ SIGNATURES = {
     "uppe3d_adjoint_workspace_size": (c_int, [P(Uppe3DDesc), P(c_size_t)]),
     "uppe3d_adjoint_step": (c_int, [c_void_p, c_double] + [c_void_p] * 7),
     "uppe3d_adjoint_check": (c_int, [c_void_p, c_void_p, P(c_int)]),
+    "uppe_rf3d_workspace_size": (c_int, [P(UppeRf3DDesc), P(c_size_t)]),
+    "uppe_rf3d_plan_create": (c_int, [P(UppeRf3DDesc), c_void_p, c_size_t] + [c_void_p] * 7
+                              + [c_void_p, P(c_void_p)]),
+    "uppe_rf3d_plan_destroy": (c_int, [c_void_p]),
+    "uppe_rf3d_load": (c_int, [c_void_p, c_void_p, c_double, c_void_p]),
+    "uppe_rf3d_attempt": (c_int, [c_void_p, c_double, c_void_p, P(c_double)]),
+    "uppe_rf3d_accept": (c_int, [c_void_p]),
+    "uppe_rf3d_store_field": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "uppe_rf3d_store_spectrum": (c_int, [c_void_p, c_void_p, c_void_p]),
     "uppe1d_workspace_size": (c_int, [P(Uppe1DDesc), P(c_size_t)]),
     "uppe1d_propagate": (c_int, [P(Uppe1DDesc)] + [c_void_p] * 5
                          + [c_void_p] * 5 + [P(Uppe1DRecord), c_void_p, P(Uppe1DStatus)]),

@@ -7,7 +7,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = [os.path.join(HERE, "csrc", f) for f in ("uppe3d.hip", "uppe1d.hip")
+SRC = [os.path.join(HERE, "csrc", f) for f in ("uppe3d.hip", "uppe1d.hip", "rf3d.hip")
        if os.path.exists(os.path.join(HERE, "csrc", f))]
 OUT = os.path.join(HERE, "libuppe.so")
 FLAGS = [
@@ -18,16 +18,33 @@ FLAGS = [
 ]
 
 
-def build(verbose=False):
-    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    tmp = OUT + ".tmp"
-    cmd = [hipcc] + FLAGS + SRC + ["-o", tmp]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True)
+    return subprocess.run(cmd, capture_output=True, text=True)
+
+
+def build(verbose=False):
+    """Compiles every source to an object in parallel (no cross-file device calls, so no
+    relocatable device code), then links the shared library."""
+    from concurrent.futures import ThreadPoolExecutor
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    objdir = os.path.join(HERE, "_obj")
+    os.makedirs(objdir, exist_ok=True)
+    objs = [os.path.join(objdir, os.path.basename(f) + ".o") for f in SRC]
+    compile_flags = [f for f in FLAGS if f != "-shared"]
+    cmds = [[hipcc] + compile_flags + ["-c", src, "-o", obj] for src, obj in zip(SRC, objs)]
+    with ThreadPoolExecutor(max_workers=len(cmds)) as ex:
+        results = list(ex.map(lambda c: _run(c, verbose), cmds))
+    for r in results:
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("hipcc failed compiling a libuppe.so source")
+    tmp = OUT + ".tmp"
+    r = _run([hipcc] + FLAGS + objs + ["-o", tmp], verbose)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libuppe.so")
+        raise RuntimeError("hipcc failed linking libuppe.so")
     os.replace(tmp, OUT)
     return OUT
 
