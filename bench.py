@@ -335,6 +335,9 @@ def main():
                     help="oracle attempts timed for the CPU baseline (0 skips it)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the 1D, batched and adjoint legs")
+    ap.add_argument("--only", choices=["mpc_3d", "realfield_3d", "cavity_1d", "batched_1d",
+                                       "config0_1d", "adjoint_3d"],
+                    help="run one side leg alone and print its JSON (not the headline line)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="before the run, copy a MIB-MiB float32 buffer 3 times (a plain copy "
                          "kernel of known traffic, to scale HBM byte counters)")
@@ -343,6 +346,12 @@ def main():
         print("bench.py: this build measures one GPU only (the 128x128x1024 field is not "
               "sharded; see the sharding design notes). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.only:   # one side leg alone (for profiling it); prints that leg's JSON
+        legs = {"mpc_3d": bench_mpc, "realfield_3d": bench_realfield,
+                "cavity_1d": lambda: bench_cavity(cpu=False), "batched_1d": bench_batched,
+                "config0_1d": lambda: bench_config0(cpu=False), "adjoint_3d": bench_adjoint3d}
+        print(json.dumps({args.only: legs[args.only]()}), flush=True)
+        return
 
     import torch
     from project_amd import _lib, media_physics as mp, workloads as wl
