@@ -16,7 +16,7 @@
 //   err = max|E4 - E_low| / (max|E4| + 1e-20);  E' = exp(-damp h) irFFT_t(filter iFFT2(E4))
 // with L = exp(h/2 D).  The controller (accept / reject, solver_3d.py:189-223) runs on the host.
 //
-// Kernels (first version, not fused): radix-2 Stockham line transforms through LDS (one
+// Kernels (first version, not fused): radix-4 Stockham line transforms through LDS (one
 // workgroup per t line; tiles of 16 adjacent omega for the x and y transforms, so every HBM
 // row is 128 contiguous bytes), one thread per (x, y) line for the serial plasma / Drude
 // recurrence (FP64, as the reference kernel), and elementwise stage kernels.
@@ -39,33 +39,53 @@ constexpr int LT = 256;   // threads of the line and tile kernels
 constexpr int G = 16;     // omega columns per xy tile
 constexpr size_t kAlign = 256;
 
-// Radix-2 Stockham FFT of line g (length N, stride LD between lines) held in LDS buffer a,
-// scratch b, with `per` threads per line (lid = 0..per-1).  Stage size ns: input
-// x[j + r N/2] (r = 0, 1) -> output y[(j / ns) 2 ns + j % ns + r ns], twiddle W_N^(k N/(2 ns))
-// on r = 1 (W_N^m = exp(-2 pi i m / N) in tw; DIR > 0 forward, DIR < 0 inverse).  Every
-// stage ends with a workgroup barrier.  Returns the buffer (a or b) that holds the result.
+// Stockham FFT of line g (length N, stride LD between lines) held in LDS buffer a, scratch b,
+// with `per` threads per line (lid = 0..per-1): radix-4 stages, then one radix-2 stage when
+// log2 N is odd.  Stage of size ns and radix R: input x[j + r N/R] (r < R), twiddled by
+// W_N^(k r N/(R ns)) with k = j % ns (W_N^m = exp(-2 pi i m / N) in tw; DIR > 0 forward,
+// DIR < 0 inverse), an R-point DFT, output y[(j / ns) R ns + k + r ns].  Every stage ends
+// with a workgroup barrier.  Returns the buffer (a or b) that holds the result.
 template <int N, int DIR>
 __device__ float2 *group_fft(float2 *a, float2 *b, int LD, int g, int lid, int per,
                              const float2 *__restrict__ tw) {
     float2 *src = a, *dst = b;
-    for (int ns = 1; ns < N; ns <<= 1) {
+    int ns = 1;
+    for (; ns * 4 <= N; ns <<= 2) {
         float2 *s = src + g * LD, *d = dst + g * LD;
-        for (int j = lid; j < N / 2; j += per) {
+        for (int j = lid; j < N / 4; j += per) {
             const int k = j & (ns - 1);
-            const float2 u0 = s[j];
-            float2 u1 = s[j + N / 2];
+            float2 v[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) v[r] = s[j + r * (N / 4)];
             if (ns > 1) {
-                const float2 w = tw[k * (N / (2 * ns))];
-                u1 = DIR > 0 ? cmul(u1, w) : cmulc(u1, w);
+                const int st = k * (N / (4 * ns));
+#pragma unroll
+                for (int r = 1; r < 4; ++r) {
+                    const float2 w = tw[st * r];
+                    v[r] = DIR > 0 ? cmul(v[r], w) : cmulc(v[r], w);
+                }
             }
-            const int o = ((j - k) << 1) + k;
-            d[o] = cadd(u0, u1);
-            d[o + ns] = csub(u0, u1);
+            Dft<4, DIR>::run(v);
+            const int o = (j - k) * 4 + k;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) d[o + r * ns] = v[r];
         }
         __syncthreads();
         float2 *t = src;
         src = dst;
         dst = t;
+    }
+    if (ns < N) {   // N = 2 ns
+        float2 *s = src + g * LD, *d = dst + g * LD;
+        for (int j = lid; j < N / 2; j += per) {
+            const float2 u0 = s[j];
+            const float2 w = tw[j];
+            const float2 u1 = DIR > 0 ? cmul(s[j + N / 2], w) : cmulc(s[j + N / 2], w);
+            d[j] = cadd(u0, u1);
+            d[j + ns] = csub(u0, u1);
+        }
+        __syncthreads();
+        src = dst;
     }
     return src;
 }
@@ -158,17 +178,21 @@ __global__ __launch_bounds__(LT) void xy_kernel(const float2 *src, float2 *dst, 
         for (int i = i0; i < N; i += PER) dst[base + (size_t)i * stride + g] = r[g * LD + i];
 }
 
+constexpr int LUT_MAX = 4096;   // largest rate table accepted
+constexpr int RCH = 16;         // t samples loaded ahead per current-kernel chunk
+
 struct RfPhys {
     float i_coef, chi3c, chi5c, av_coef, e_scale;
     float lut_x0, lut_x1;
     int lut_n, use_chi5, use_av;
     double lut_dx;
     double rate_rec, dt, rho_nt, Ui, e2m, nu_c, exp_nu, step_J;
+    double eg0, sf0;   // exp(gdt) and its step factor without avalanche (gdt = -rate_rec dt)
 };
 
 // np.interp of the float32 table (x ascending, nearly uniform) at float32 lg, in FP64.
-__device__ float lut_interp(float lg, const float *__restrict__ lx, const float *__restrict__ lw,
-                            const RfPhys &ph) {
+__device__ __forceinline__ float lut_interp(float lg, const float *lx, const float *lw,
+                                            const RfPhys &ph) {
     lg = fminf(fmaxf(lg, ph.lut_x0), ph.lut_x1);
     int j = (int)(((double)lg - (double)ph.lut_x0) / ph.lut_dx);
     j = j < 0 ? 0 : (j > ph.lut_n - 2 ? ph.lut_n - 2 : j);
@@ -179,48 +203,74 @@ __device__ float lut_interp(float lg, const float *__restrict__ lx, const float 
     return (float)(y0 + ((double)lg - x0) * (y1 - y0) / (x1 - x0));
 }
 
-// media_physics.py:258-318 with the plasma kernel of :11-118: one thread per (x, y) line,
-// serial in t.  E: field lines (normalised), times e_scale (float32) = the physical field.
-// Writes P / e_scale and J / e_scale (float32).
-__global__ __launch_bounds__(64) void response_kernel(const float *__restrict__ En, float *P,
-                                                      float *J, RfPhys ph,
-                                                      const float *__restrict__ lx,
-                                                      const float *__restrict__ lw, int nlines,
-                                                      int nt, float in_scale) {
-    const int line = blockIdx.x * blockDim.x + threadIdx.x;
-    if (line >= nlines) return;
-    const size_t base = (size_t)line * nt;
-    double rho = 0.0, jf = 0.0;
-    const double rnt = ph.rho_nt;
-    for (int t = 0; t < nt; ++t) {
-        const float Ef = En[base + t] * in_scale;
+// media_physics.py:258-318, the sample-local part: per sample (all in parallel) the
+// ionisation rate from the table and the polarisation P = eps0 (chi3 E^3 + chi5 E^5) / e_scale.
+// E: field lines (normalised), times e_scale (float32) = the physical field.
+__global__ __launch_bounds__(256) void rates_kernel(const float *__restrict__ En, float *P,
+                                                   float *Wi, RfPhys ph,
+                                                   const float *__restrict__ lx,
+                                                   const float *__restrict__ lw, size_t n,
+                                                   float in_scale) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * 256) {
+        const float Ef = En[i] * in_scale;
         const float I = ph.i_coef * (Ef * Ef);
-        const float wi = lut_interp(log10f(I + 1e-30f), lx, lw, ph);
-        const float wa = ph.use_av ? ph.av_coef * I : 0.0f;
-        const double E = (double)Ef, w_pi = (double)wi, w_av = (double)wa;
-        double dep = (rnt - rho) / rnt;
-        if (dep < 0.0) dep = 0.0;
-        const double S = w_pi * rnt * dep;
-        const double Gm = w_av * dep - ph.rate_rec;
-        double gdt = Gm * ph.dt;
-        gdt = gdt > 10.0 ? 10.0 : (gdt < -10.0 ? -10.0 : gdt);
-        const double eg = exp(gdt);
-        const double sf = fabs(gdt) < 1e-5 ? ph.dt * (1.0 + 0.5 * gdt) : (eg - 1.0) / Gm;
-        double nr = rho * eg + S * sf;
-        nr = nr > rnt ? rnt : (nr < 0.0 ? 0.0 : nr);
-        const double gen = S + w_av * dep * rho;
-        const double nj = jf * ph.exp_nu + ph.e2m * rho * E * ph.step_J;
-        const double jl = fabs(E) > 1e-10 ? (gen * ph.Ui * E) / (E * E + 1e-20) : 0.0;
-        const float jout = (float)(jf + jl);
+        Wi[i] = lut_interp(log10f(I + 1e-30f), lx, lw, ph);
         float p = ((ph.chi3c * Ef) * Ef) * Ef;
         if (ph.use_chi5) {
             const float e2 = Ef * Ef;
             p = p + ((ph.chi5c * Ef) * e2) * e2;
         }
-        P[base + t] = p / ph.e_scale;
-        J[base + t] = jout / ph.e_scale;
-        rho = nr;
-        jf = nj;
+        P[i] = p / ph.e_scale;
+    }
+}
+
+// The plasma kernel of media_physics.py:11-118: one thread per (x, y) line, serial in t (the
+// density with depletion and the Drude current are recurrences); FP64 as the reference.
+// Reads the field and the rate of rates_kernel RCH samples ahead; writes J / e_scale.
+__global__ __launch_bounds__(64) void current_kernel(const float *__restrict__ En,
+                                                     const float *__restrict__ Wi, float *J,
+                                                     RfPhys ph, int nlines, int nt,
+                                                     float in_scale) {
+    const int line = blockIdx.x * blockDim.x + threadIdx.x;
+    if (line >= nlines) return;
+    const size_t base = (size_t)line * nt;
+    double rho = 0.0, jf = 0.0;
+    const double rnt = ph.rho_nt;
+    for (int t0 = 0; t0 < nt; t0 += RCH) {
+        float ev[RCH], wv[RCH], jv[RCH];
+#pragma unroll
+        for (int k = 0; k < RCH; ++k) {
+            ev[k] = En[base + t0 + k];
+            wv[k] = Wi[base + t0 + k];
+        }
+#pragma unroll
+        for (int k = 0; k < RCH; ++k) {
+            const float Ef = ev[k] * in_scale;
+            const float wa = ph.use_av ? ph.av_coef * (ph.i_coef * (Ef * Ef)) : 0.0f;
+            const double E = (double)Ef, w_pi = (double)wv[k], w_av = (double)wa;
+            double dep = (rnt - rho) / rnt;
+            if (dep < 0.0) dep = 0.0;
+            const double S = w_pi * rnt * dep;
+            double eg = ph.eg0, sf = ph.sf0;
+            if (ph.use_av) {
+                const double Gm = w_av * dep - ph.rate_rec;
+                double gdt = Gm * ph.dt;
+                gdt = gdt > 10.0 ? 10.0 : (gdt < -10.0 ? -10.0 : gdt);
+                eg = exp(gdt);
+                sf = fabs(gdt) < 1e-5 ? ph.dt * (1.0 + 0.5 * gdt) : (eg - 1.0) / Gm;
+            }
+            double nr = rho * eg + S * sf;
+            nr = nr > rnt ? rnt : (nr < 0.0 ? 0.0 : nr);
+            const double gen = S + w_av * dep * rho;
+            const double nj = jf * ph.exp_nu + ph.e2m * rho * E * ph.step_J;
+            const double jl = fabs(E) > 1e-10 ? (gen * ph.Ui * E) / (E * E + 1e-20) : 0.0;
+            jv[k] = (float)(jf + jl) / ph.e_scale;
+            rho = nr;
+            jf = nj;
+        }
+#pragma unroll
+        for (int k = 0; k < RCH; ++k) J[base + t0 + k] = jv[k];
     }
 }
 
@@ -307,7 +357,7 @@ __global__ void load_kernel(const float *in, float *out, float s, size_t n) {
 }
 
 struct Layout {
-    size_t et, enew, etmp, pb, jb;                 // real [lines][nt]
+    size_t et, enew, etmp, pb, jb, wib;            // real [lines][nt]
     size_t S, v, a1, low, x, nout, Y, D, Pc, Jc;   // complex [lines][W]
     size_t filt, damp, twt, twx, twy, lx, lw, maxb, total;
 };
@@ -327,6 +377,7 @@ Layout layout(const uppe_rf3d_desc *d) {
     L.etmp = take(nr);
     L.pb = take(nr);
     L.jb = take(nr);
+    L.wib = take(nr);
     L.S = take(nc);
     L.v = take(nc);
     L.a1 = take(nc);
@@ -357,7 +408,8 @@ int check_desc(const uppe_rf3d_desc *d) {
                d->nt);
     UPPE_CHECK(pow2_in(d->nx, 16, 256) && pow2_in(d->ny, 16, 256),
                "uppe_rf3d: nx, ny (%d, %d) must be powers of two in 16..256", d->nx, d->ny);
-    UPPE_CHECK(d->lut_n >= 2, "uppe_rf3d: lut_n must be >= 2");
+    UPPE_CHECK(d->lut_n >= 2 && d->lut_n <= LUT_MAX, "uppe_rf3d: lut_n must be in 2..%d",
+               LUT_MAX);
     UPPE_CHECK(d->dt > 0.0 && d->rho_nt > 0.0 && d->tau_c > 0.0,
                "uppe_rf3d: dt, rho_nt and tau_c must be positive");
     return UPPE_OK;
@@ -384,6 +436,11 @@ struct uppe_rf3d_plan {
 };
 
 namespace {
+
+int stage_grid(size_t n) {
+    const size_t g = (n + 255) / 256;
+    return (int)(g < 4096 ? g : 4096);
+}
 
 template <int NT>
 void launch_line(int which, uppe_rf3d_plan *p, hipStream_t s, const void *in, void *out,
@@ -453,18 +510,16 @@ void nonlinear(uppe_rf3d_plan *p, hipStream_t s, const float2 *X, float2 *out) {
     xy_pass<-1>(p, s, Y, Y, 0);
     const float inv = (float)(1.0 / ((double)p->d.nt * p->d.nx * p->d.ny));
     line_irfft(p, s, Y, etmp, nullptr, nullptr, 0.0f, inv);
-    response_kernel<<<(lines + 63) / 64, 64, 0, s>>>(
-        etmp, (float *)(p->ws + p->L.pb), (float *)(p->ws + p->L.jb), p->ph,
-        (const float *)(p->ws + p->L.lx), (const float *)(p->ws + p->L.lw), lines, p->d.nt,
-        p->e_scale);
+    const size_t nr = (size_t)lines * p->d.nt;
+    float *wib = (float *)(p->ws + p->L.wib);
+    rates_kernel<<<stage_grid(nr), 256, 0, s>>>(etmp, (float *)(p->ws + p->L.pb), wib, p->ph,
+                                                 (const float *)(p->ws + p->L.lx),
+                                                 (const float *)(p->ws + p->L.lw), nr, p->e_scale);
+    current_kernel<<<(lines + 63) / 64, 64, 0, s>>>(etmp, wib, (float *)(p->ws + p->L.jb), p->ph,
+                                                    lines, p->d.nt, p->e_scale);
     line_source(p, s, Y);
     xy_pass<1>(p, s, Y, out, 1);
     xy_pass<1>(p, s, out, out, 0);
-}
-
-int stage_grid(size_t n) {
-    const size_t g = (n + 255) / 256;
-    return (int)(g < 4096 ? g : 4096);
 }
 
 }  // namespace
@@ -544,6 +599,14 @@ int uppe_rf3d_plan_create(const uppe_rf3d_desc *d, void *workspace, size_t works
     const double nudt = ph.nu_c * ph.dt;
     ph.exp_nu = exp(-nudt);
     ph.step_J = fabs(nudt) < 1e-5 ? ph.dt * (1.0 - 0.5 * nudt) : (1.0 - ph.exp_nu) / ph.nu_c;
+    {
+        // without avalanche gdt = -rate_rec dt at every sample: the kernel's formula, once
+        const double Gm = -ph.rate_rec;
+        double gdt = Gm * ph.dt;
+        gdt = gdt > 10.0 ? 10.0 : (gdt < -10.0 ? -10.0 : gdt);
+        ph.eg0 = exp(gdt);
+        ph.sf0 = fabs(gdt) < 1e-5 ? ph.dt * (1.0 + 0.5 * gdt) : (ph.eg0 - 1.0) / Gm;
+    }
     *out = p;
     return UPPE_OK;
 }
