@@ -212,7 +212,7 @@ def bench_mpc(steps=60):
             "projected_cycle_s_plates": round(plates * per, 1)}
 
 
-def bench_realfield(nx=128, nt=1024, attempts=10):
+def bench_realfield(nx=128, nt=1024, attempts=10, cpu=True):
     """Real-field solver (src/3+1D-UPPE-ERK4(3)IP-real-field/): argon 1 atm, 800 nm, 30 fs,
     3e14 W/cm^2 peak, nx x nx x nt over 2 mm and 300 fs; `attempts` attempts of a fixed step."""
     import torch
@@ -234,9 +234,21 @@ def bench_realfield(nx=128, nt=1024, attempts=10):
     for _ in range(attempts):
         s.attempt(1e-6)
     wall = time.perf_counter() - t0
-    return {"workload": "real-field 3+1D, argon, 800 nm 30 fs 3e14 W/cm^2, %dx%dx%d" % (nx, nx, nt),
-            "attempts": attempts, "ms_per_attempt": round(1e3 * wall / attempts, 3),
-            "attempts_per_s": round(attempts / wall, 1)}
+    out = {"workload": "real-field 3+1D, argon, 800 nm 30 fs 3e14 W/cm^2, %dx%dx%d" % (nx, nx, nt),
+           "attempts": attempts, "ms_per_attempt": round(1e3 * wall / attempts, 3),
+           "attempts_per_s": round(attempts / wall, 1)}
+    if cpu:
+        from oracle import realfield as orf
+        o = orf.OracleRF(x, x, t, orf.RFMedium("Argon", 800e-9))
+        o.E_scale = float(np.max(np.abs(E)))
+        En = E.astype(np.float64) / o.E_scale
+        t0 = time.perf_counter()
+        o.step(En, 1e-6)
+        cs = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value_s_per_attempt": round(cs, 3), "cores": 1, "kind": "port",
+                               "sample": "one attempt of the same step with the FP64 oracle"}
+        out["speedup_vs_cpu"] = round(cs / (wall / attempts), 1)
+    return out
 
 
 def bench_adjoint():
@@ -347,7 +359,7 @@ def main():
               "sharded; see the sharding design notes). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     if args.only:   # one side leg alone (for profiling it); prints that leg's JSON
-        legs = {"mpc_3d": bench_mpc, "realfield_3d": bench_realfield,
+        legs = {"mpc_3d": bench_mpc, "realfield_3d": lambda: bench_realfield(cpu=False),
                 "cavity_1d": lambda: bench_cavity(cpu=False), "batched_1d": bench_batched,
                 "config0_1d": lambda: bench_config0(cpu=False), "adjoint_3d": bench_adjoint3d}
         print(json.dumps({args.only: legs[args.only]()}), flush=True)
@@ -457,7 +469,8 @@ def main():
         out["config0_1d"] = bench_config0(cpu=args.cpu_attempts > 0)
         out["batched_1d"] = bench_batched()
         out["cavity_1d"] = bench_cavity(cpu=args.cpu_attempts > 0)
-        for key, leg in (("mpc_3d", bench_mpc), ("realfield_3d", bench_realfield)):
+        for key, leg in (("mpc_3d", bench_mpc),
+                         ("realfield_3d", lambda: bench_realfield(cpu=args.cpu_attempts > 0))):
             try:
                 out[key] = leg()
             except Exception as e:   # a failing side leg is reported, not fatal to the line

@@ -225,16 +225,11 @@ __global__ __launch_bounds__(256) void rates_kernel(const float *__restrict__ En
     }
 }
 
-// The plasma kernel of media_physics.py:11-118: one thread per (x, y) line, serial in t (the
-// density with depletion and the Drude current are recurrences); FP64 as the reference.
-// Reads the field and the rate of rates_kernel RCH samples ahead; writes J / e_scale.
-__global__ __launch_bounds__(64) void current_kernel(const float *__restrict__ En,
-                                                     const float *__restrict__ Wi, float *J,
-                                                     RfPhys ph, int nlines, int nt,
-                                                     float in_scale) {
-    const int line = blockIdx.x * blockDim.x + threadIdx.x;
-    if (line >= nlines) return;
-    const size_t base = (size_t)line * nt;
+// The plasma kernel of media_physics.py:11-118 for one (x, y) line, serial in t (the density
+// with depletion and the Drude current are recurrences); FP64 as the reference.  Reads the
+// field and the rate of rates_kernel RCH samples ahead; writes J / e_scale.
+__device__ void current_serial(const float *__restrict__ En, const float *__restrict__ Wi,
+                               float *J, const RfPhys &ph, size_t base, int nt, float in_scale) {
     double rho = 0.0, jf = 0.0;
     const double rnt = ph.rho_nt;
     for (int t0 = 0; t0 < nt; t0 += RCH) {
@@ -271,6 +266,75 @@ __global__ __launch_bounds__(64) void current_kernel(const float *__restrict__ E
         }
 #pragma unroll
         for (int k = 0; k < RCH; ++k) J[base + t0 + k] = jv[k];
+    }
+}
+
+// One thread per line (media with avalanche: the density recurrence is nonlinear).
+__global__ __launch_bounds__(64) void current_kernel(const float *__restrict__ En,
+                                                     const float *__restrict__ Wi, float *J,
+                                                     RfPhys ph, int nlines, int nt,
+                                                     float in_scale) {
+    const int line = blockIdx.x * blockDim.x + threadIdx.x;
+    if (line >= nlines) return;
+    current_serial(En, Wi, J, ph, (size_t)line * nt, nt, in_scale);
+}
+
+// Inclusive scan of affine maps x -> a x + b over the 64 lanes: lane l ends with
+// map_l o ... o map_0.
+__device__ __forceinline__ void scan_affine(double &a, double &b, int lane) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const double pa = __shfl_up(a, o, 64), pb = __shfl_up(b, o, 64);
+        if (lane >= o) {
+            b = a * pb + b;
+            a = a * pa;
+        }
+    }
+}
+
+// Media without avalanche: one wave per line, 64 samples per step of the loop.  With
+// gdt = -rate_rec dt at every sample the density update is the affine map
+// rho -> (e - w sf) rho + w rho_nt sf (e, sf: eg0, sf0), which keeps rho in [0, rho_nt] when
+// w sf <= e, so the reference's clamps never act; the Drude current is affine in its state
+// (rho -> jf e_nu + e2m rho E sJ).  Both are prefix-scanned over the wave.  A line with some
+// w sf > e runs the serial recurrence on lane 0.
+__global__ __launch_bounds__(256) void current_scan_kernel(const float *__restrict__ En,
+                                                           const float *__restrict__ Wi,
+                                                           float *J, RfPhys ph, int nlines,
+                                                           int nt, float in_scale) {
+    const int lane = threadIdx.x & 63;
+    const int line = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (line >= nlines) return;   // uniform per wave; no workgroup barrier below
+    const size_t base = (size_t)line * nt;
+    const double rnt = ph.rho_nt;
+    int ok = 1;
+    for (int t = lane; t < nt; t += 64) ok &= ((double)Wi[base + t] * ph.sf0 <= ph.eg0) ? 1 : 0;
+    if (!__all(ok)) {
+        if (lane == 0) current_serial(En, Wi, J, ph, base, nt, in_scale);
+        return;
+    }
+    double rho0 = 0.0, jf0 = 0.0;
+    for (int t0 = 0; t0 < nt; t0 += 64) {
+        const size_t e = base + t0 + lane;
+        const float Ef = En[e] * in_scale;
+        const double E = (double)Ef, w = (double)Wi[e];
+        double a = ph.eg0 - w * ph.sf0, b = w * rnt * ph.sf0;
+        scan_affine(a, b, lane);
+        const double rho_next = a * rho0 + b;
+        double rho = __shfl_up(rho_next, 1, 64);
+        if (lane == 0) rho = rho0;
+        double dep = (rnt - rho) / rnt;
+        if (dep < 0.0) dep = 0.0;
+        const double S = w * rnt * dep;
+        double ja = ph.exp_nu, jb = ph.e2m * rho * E * ph.step_J;
+        scan_affine(ja, jb, lane);
+        const double jf_next = ja * jf0 + jb;
+        double jf = __shfl_up(jf_next, 1, 64);
+        if (lane == 0) jf = jf0;
+        const double jl = fabs(E) > 1e-10 ? (S * ph.Ui * E) / (E * E + 1e-20) : 0.0;
+        J[e] = (float)(jf + jl) / ph.e_scale;
+        rho0 = __shfl(rho_next, 63, 64);
+        jf0 = __shfl(jf_next, 63, 64);
     }
 }
 
@@ -515,8 +579,13 @@ void nonlinear(uppe_rf3d_plan *p, hipStream_t s, const float2 *X, float2 *out) {
     rates_kernel<<<stage_grid(nr), 256, 0, s>>>(etmp, (float *)(p->ws + p->L.pb), wib, p->ph,
                                                  (const float *)(p->ws + p->L.lx),
                                                  (const float *)(p->ws + p->L.lw), nr, p->e_scale);
-    current_kernel<<<(lines + 63) / 64, 64, 0, s>>>(etmp, wib, (float *)(p->ws + p->L.jb), p->ph,
-                                                    lines, p->d.nt, p->e_scale);
+    float *jb = (float *)(p->ws + p->L.jb);
+    if (p->ph.use_av)
+        current_kernel<<<(lines + 63) / 64, 64, 0, s>>>(etmp, wib, jb, p->ph, lines, p->d.nt,
+                                                        p->e_scale);
+    else
+        current_scan_kernel<<<(lines + 3) / 4, 256, 0, s>>>(etmp, wib, jb, p->ph, lines,
+                                                            p->d.nt, p->e_scale);
     line_source(p, s, Y);
     xy_pass<1>(p, s, Y, out, 1);
     xy_pass<1>(p, s, out, out, 0);
