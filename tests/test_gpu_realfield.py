@@ -71,6 +71,31 @@ def test_replay_and_free_running(gpu, case):
         assert rel_l2(Ew2, Ew_o) < 10 * bound
 
 
+SIZES = [
+    # every line-transform length and xy-transform length the kernels instantiate at least once
+    # (nx, ny, nt, T): T keeps the 800 nm carrier below 0.85 of the Nyquist frequency
+    (16, 16, 64, 60e-15),
+    (16, 16, 2048, 1200e-15),
+    (256, 16, 128, 120e-15),
+    (32, 64, 512, 400e-15),
+    (64, 128, 256, 200e-15),
+]
+
+
+@pytest.mark.parametrize("nx,ny,nt,T", SIZES, ids=["%dx%dx%d" % c[:3] for c in SIZES])
+def test_every_kernel_size(gpu, nx, ny, nt, T):
+    om = rf.RFMedium("Argon", 800e-9, 1.0)
+    x, y, t, E = field(nx, ny, nt, T, 800e-9, 2e18, om.n0, tau=min(20e-15, T / 6))
+    o = rf.OracleRF(x, y, t, om)
+    s = ERK43_RealField_UPPE_3D(x, y, t, mr.Argon(800e-9, 1.0))
+    z_o, f_o, Ew_o, log = o.propagate(E, 0.2e-3, tol=1e-4, max_attempts=6)
+    announce("realfield size %dx%dx%d attempts=%d" % (nx, ny, nt, len(log)))
+    _, _, Ew_g = s.propagate_replay(E, log)
+    e = rel_l2(Ew_g, Ew_o)
+    print("rel L2", e)
+    assert e < 2e-5
+
+
 def test_field_roundtrip_and_errors(gpu):
     x, y, t, E = field(16, 16, 128, 150e-15, 800e-9, 1e16, 1.0)
     s = ERK43_RealField_UPPE_3D(x, y, t, mr.Argon(800e-9))
