@@ -175,14 +175,19 @@ typedef struct {
     double av_coef;     /* avalanche rate = av_coef I (0: none) */
     double rate_rec;    /* 1 / tau_rec */
     double rho_nt, Ui, e_charge, m_e, tau_c;   /* SI; rounded to float32 like the reference */
+    double coupling;    /* 2 eps0 c^2 (the P / J coefficients' denominator is coupling kz) */
 } uppe_rf3d_desc;
 typedef struct uppe_rf3d_plan uppe_rf3d_plan;
 int uppe_rf3d_workspace_size(const uppe_rf3d_desc *d, size_t *bytes);
-/* D, Pc, Jc: host complex64 [nx][ny][nt/2+1] (solver_3d.py:54-100: the propagator exponent
- * and the P / J coupling coefficients); filt: float32 [nt/2+1]; damp: float32 [nx][ny];
- * lut_x, lut_w: the float32 log10(I) grid and PPT rates (media_physics.py:167-183). */
+/* Host FP64 tables over omega_w = 2 pi rfftfreq (nt/2+1 each): k_w = n(omega) omega / c (omega
+ * = 0 replaced by 1e-12), kref_w = k1 omega; and kx2 [nx], ky2 [ny] (squared transverse
+ * wavenumbers).  From them the kernels form the propagator exponent -i (kz - kref) and the
+ * P / J coupling coefficients of solver_3d.py:54-100 per element (rounded to complex64 as the
+ * reference's arrays).  filt: float32 [nt/2+1]; damp: float32 [nx][ny]; lut_x, lut_w: the
+ * float32 log10(I) grid and PPT rates (media_physics.py:167-183). */
 int uppe_rf3d_plan_create(const uppe_rf3d_desc *d, void *workspace, size_t workspace_bytes,
-                          const void *D, const void *Pc, const void *Jc, const float *filt,
+                          const double *k_w, const double *kref_w, const double *omega_w,
+                          const double *kx2, const double *ky2, const float *filt,
                           const float *damp, const float *lut_x, const float *lut_w,
                           void *stream, uppe_rf3d_plan **out);
 int uppe_rf3d_plan_destroy(uppe_rf3d_plan *p);

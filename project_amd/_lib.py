@@ -34,7 +34,8 @@ class UppeRf3DDesc(ctypes.Structure):
     _fields_ = [("nx", c_int), ("ny", c_int), ("nt", c_int), ("lut_n", c_int), ("dt", c_double),
                 ("i_coef", c_double), ("chi3_coef", c_double), ("chi5_coef", c_double),
                 ("av_coef", c_double), ("rate_rec", c_double), ("rho_nt", c_double),
-                ("Ui", c_double), ("e_charge", c_double), ("m_e", c_double), ("tau_c", c_double)]
+                ("Ui", c_double), ("e_charge", c_double), ("m_e", c_double), ("tau_c", c_double),
+                ("coupling", c_double)]
 
 
 class Uppe1DDesc(ctypes.Structure):
@@ -97,7 +98,7 @@ SIGNATURES = {
     "uppe3d_adjoint_step": (c_int, [c_void_p, c_double] + [c_void_p] * 7),
     "uppe3d_adjoint_check": (c_int, [c_void_p, c_void_p, P(c_int)]),
     "uppe_rf3d_workspace_size": (c_int, [P(UppeRf3DDesc), P(c_size_t)]),
-    "uppe_rf3d_plan_create": (c_int, [P(UppeRf3DDesc), c_void_p, c_size_t] + [c_void_p] * 7
+    "uppe_rf3d_plan_create": (c_int, [P(UppeRf3DDesc), c_void_p, c_size_t] + [c_void_p] * 9
                               + [c_void_p, P(c_void_p)]),
     "uppe_rf3d_plan_destroy": (c_int, [c_void_p]),
     "uppe_rf3d_load": (c_int, [c_void_p, c_void_p, c_double, c_void_p]),

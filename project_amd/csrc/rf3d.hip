@@ -4,9 +4,10 @@
 // (t -> omega >= 0, W = N_t/2 + 1 bins; (x, y) -> (kx, ky)).
 //
 // Layouts: real lines [x][y][t] (N_t floats per line); spectra [x][y][w] (W complex64 per line;
-// after the xy transforms the same array is [kx][ky][w]).  The operator tables D, P-coefficient
-// and J-coefficient are full [x][y][w] complex64 arrays computed on the host in FP64, as the
-// reference stores them.
+// after the xy transforms the same array is [kx][ky][w]).  The reference stores the propagator
+// exponent D and the P / J coupling coefficients as full [x][y][w] complex64 arrays; here they
+// are formed per element from FP64 tables of k(omega), k1 omega, omega, kx^2 and ky^2 (same
+// formulas and rounding to complex64), which saves three array reads per use.
 //
 // One attempt, all kernels on one stream (the host reads the two error maxima at its end):
 //   S = FFT2(rFFT_t(E))                                   (state in (kx, ky, w))
@@ -132,14 +133,76 @@ __global__ __launch_bounds__(LT) void irfft_kernel(const float2 *__restrict__ in
     for (int t = tid; t < NT; t += LT) dst[t] = (r[t].x * scale) * f;
 }
 
+// Per-omega and per-axis FP64 tables from which the propagator exponent D and the P / J
+// coupling coefficients are formed element by element (solver_3d.py:54-100), in the host
+// formulas' order of operations, then rounded to float as the reference's complex64 arrays.
+struct Ops {
+    const double *k, *kref, *om, *kx2, *ky2;   // k(omega), k1 omega, omega; kx^2, ky^2
+    double cpl;                                 // 2 eps0 c^2
+    int ny, W;
+};
+
+// kz = sqrt(k^2 - kperp^2), or -i sqrt|.| for evanescent components
+__device__ __forceinline__ void kz_at(const Ops &o, double K2, int w, double &zr, double &zi) {
+    const double kz2 = o.k[w] * o.k[w] - K2;
+    if (kz2 >= 0.0) {
+        zr = sqrt(kz2);
+        zi = 0.0;
+    } else {
+        zr = 0.0;
+        zi = -sqrt(-kz2);
+    }
+}
+
+// D = -i (kz - k1 omega) as complex64
+__device__ __forceinline__ float2 d_at(const Ops &o, double K2, int w) {
+    double zr, zi;
+    kz_at(o, K2, w, zr, zi);
+    return make_float2((float)zi, (float)(-(zr - o.kref[w])));
+}
+
+// (ar + i ai) / (br + i bi), numpy's complex-division formula (Smith's method)
+__device__ __forceinline__ void cdiv_np(double ar, double ai, double br, double bi, double &cr,
+                                        double &ci) {
+    if (fabs(br) >= fabs(bi)) {
+        const double rat = bi / br, scl = 1.0 / (br + bi * rat);
+        cr = (ar + ai * rat) * scl;
+        ci = (ai - ar * rat) * scl;
+    } else {
+        const double rat = br / bi, scl = 1.0 / (bi + br * rat);
+        cr = (ar * rat + ai) * scl;
+        ci = (ai * rat - ar) * scl;
+    }
+}
+
+// P coefficient -i omega^2 / (2 eps0 c^2 kz) and J coefficient -omega / (2 eps0 c^2 kz),
+// |kz| < 1e-10 replaced by 1e-10, zero in the omega = 0 column
+__device__ __forceinline__ void coupling_at(const Ops &o, double K2, int w, float2 &pc,
+                                            float2 &jc) {
+    if (w == 0) {
+        pc = jc = make_float2(0.0f, 0.0f);
+        return;
+    }
+    double zr, zi;
+    kz_at(o, K2, w, zr, zi);
+    if (fabs(zr) + fabs(zi) < 1e-10) {   // one of the two is zero: |kz| = |zr| + |zi|
+        zr = 1e-10;
+        zi = 0.0;
+    }
+    const double br = o.cpl * zr, bi = o.cpl * zi, om = o.om[w];
+    double r, i;
+    cdiv_np(-0.0 * (om * om), -(om * om), br, bi, r, i);
+    pc = make_float2((float)r, (float)i);
+    cdiv_np(-om, 0.0, br, bi, r, i);
+    jc = make_float2((float)r, (float)i);
+}
+
 // N_w = Pc rFFT_t(P) + Jc rFFT_t(J) per (x, y) line (media_physics.py returns P, J; the
 // coefficient arrays are indexed like the field, solver_3d.py:121-143).
 template <int NT>
 __global__ __launch_bounds__(LT) void source_kernel(const float *__restrict__ P,
                                                     const float *__restrict__ J, float2 *out,
-                                                    const float2 *__restrict__ Pc,
-                                                    const float2 *__restrict__ Jc,
-                                                    const float2 *__restrict__ tw) {
+                                                    Ops ops, const float2 *__restrict__ tw) {
     constexpr int W = NT / 2 + 1;
     __shared__ float2 a[NT], b[NT], pw[W];
     const int tid = threadIdx.x;
@@ -152,8 +215,13 @@ __global__ __launch_bounds__(LT) void source_kernel(const float *__restrict__ P,
     for (int t = tid; t < NT; t += LT) a[t] = make_float2(J[lt + t], 0.0f);
     __syncthreads();
     r = group_fft<NT, 1>(a, b, 0, 0, tid, LT, tw);
-    for (int k = tid; k < W; k += LT)
-        out[lw + k] = cadd(cmul(Pc[lw + k], pw[k]), cmul(Jc[lw + k], r[k]));
+    const int line = blockIdx.x;
+    const double K2 = ops.kx2[line / ops.ny] + ops.ky2[line % ops.ny];
+    for (int k = tid; k < W; k += LT) {
+        float2 pc, jc;
+        coupling_at(ops, K2, k, pc, jc);
+        out[lw + k] = cadd(cmul(pc, pw[k]), cmul(jc, r[k]));
+    }
 }
 
 // One FFT pass along x or y of a [x][y][w] array: a tile is 16 adjacent omega columns of one
@@ -346,7 +414,8 @@ __device__ __forceinline__ float2 cexpf2(float2 z) {
 }
 
 struct Stage {
-    const float2 *S, *D, *nout;
+    const float2 *S, *nout;
+    Ops ops;
     float2 *v, *a1, *low, *x;
     unsigned int *maxbits;   // [0] max|E4 - E_low|^2, [1] max|E4|^2 (float bits)
     float h;
@@ -359,7 +428,9 @@ __global__ __launch_bounds__(256) void stage_kernel(Stage s, size_t n) {
     const float h = s.h, hh = 0.5f * s.h;
     unsigned int md = 0, ma = 0;
     for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
-        const float2 d = s.D[i];
+        const size_t line = i / (size_t)s.ops.W;
+        const int w = (int)(i - line * (size_t)s.ops.W);
+        const float2 d = d_at(s.ops, s.ops.kx2[line / s.ops.ny] + s.ops.ky2[line % s.ops.ny], w);
         const float2 L = cexpf2(make_float2(hh * d.x, hh * d.y));
         const float2 nv = s.nout[i];
         if (MODE == 1) {
@@ -422,7 +493,8 @@ __global__ void load_kernel(const float *in, float *out, float s, size_t n) {
 
 struct Layout {
     size_t et, enew, etmp, pb, jb, wib;            // real [lines][nt]
-    size_t S, v, a1, low, x, nout, Y, D, Pc, Jc;   // complex [lines][W]
+    size_t S, v, a1, low, x, nout, Y;              // complex [lines][W]
+    size_t kw, kref, om, kx2, ky2;                 // FP64 operator tables
     size_t filt, damp, twt, twx, twy, lx, lw, maxb, total;
 };
 
@@ -449,9 +521,11 @@ Layout layout(const uppe_rf3d_desc *d) {
     L.x = take(nc);
     L.nout = take(nc);
     L.Y = take(nc);
-    L.D = take(nc);
-    L.Pc = take(nc);
-    L.Jc = take(nc);
+    L.kw = take(8 * W);
+    L.kref = take(8 * W);
+    L.om = take(8 * W);
+    L.kx2 = take(8 * (size_t)d->nx);
+    L.ky2 = take(8 * (size_t)d->ny);
     L.filt = take(4 * W);
     L.damp = take(4 * lines);
     L.twt = take(8 * (size_t)d->nt);
@@ -496,6 +570,7 @@ struct uppe_rf3d_plan {
     char *ws;
     float *et, *enew;
     RfPhys ph;
+    Ops ops;
     float e_scale;
 };
 
@@ -538,10 +613,9 @@ void line_irfft(uppe_rf3d_plan *p, hipStream_t s, const float2 *in, float *out,
 void line_source(uppe_rf3d_plan *p, hipStream_t s, float2 *out) {
     const int lines = p->d.nx * p->d.ny;
     const float *P = (const float *)(p->ws + p->L.pb), *J = (const float *)(p->ws + p->L.jb);
-    const float2 *Pc = (const float2 *)(p->ws + p->L.Pc), *Jc = (const float2 *)(p->ws + p->L.Jc);
     const float2 *tw = (const float2 *)(p->ws + p->L.twt);
     switch (p->d.nt) {
-#define C(NT) case NT: source_kernel<NT><<<lines, LT, 0, s>>>(P, J, out, Pc, Jc, tw); break;
+#define C(NT) case NT: source_kernel<NT><<<lines, LT, 0, s>>>(P, J, out, p->ops, tw); break;
         C(64) C(128) C(256) C(512) C(1024) C(2048)
 #undef C
     }
@@ -604,13 +678,16 @@ int uppe_rf3d_workspace_size(const uppe_rf3d_desc *d, size_t *bytes) {
 }
 
 int uppe_rf3d_plan_create(const uppe_rf3d_desc *d, void *workspace, size_t workspace_bytes,
-                          const void *D, const void *Pc, const void *Jc, const float *filt,
+                          const double *k_w, const double *kref_w, const double *omega_w,
+                          const double *kx2, const double *ky2, const float *filt,
                           const float *damp, const float *lut_x, const float *lut_w,
                           void *stream, uppe_rf3d_plan **out) {
     const int rc = check_desc(d);
     if (rc != UPPE_OK) return rc;
-    UPPE_CHECK(out && workspace && D && Pc && Jc && filt && damp && lut_x && lut_w,
+    UPPE_CHECK(out && workspace && k_w && kref_w && omega_w && kx2 && ky2 && filt && damp &&
+                   lut_x && lut_w,
                "uppe_rf3d_plan_create: null argument");
+    UPPE_CHECK(d->coupling > 0.0, "uppe_rf3d_plan_create: coupling (2 eps0 c^2) must be positive");
     const Layout L = layout(d);
     UPPE_CHECK(workspace_bytes >= L.total, "uppe_rf3d_plan_create: workspace %zu < %zu bytes",
                workspace_bytes, L.total);
@@ -621,11 +698,12 @@ int uppe_rf3d_plan_create(const uppe_rf3d_desc *d, void *workspace, size_t works
     hipStream_t s = (hipStream_t)stream;
     char *ws = (char *)workspace;
     const size_t lines = (size_t)d->nx * d->ny, W = d->nt / 2 + 1;
-    const size_t nc = lines * W * 8;
     UPPE_HIP(hipStreamSynchronize(s));
-    UPPE_HIP(hipMemcpy(ws + L.D, D, nc, hipMemcpyHostToDevice));
-    UPPE_HIP(hipMemcpy(ws + L.Pc, Pc, nc, hipMemcpyHostToDevice));
-    UPPE_HIP(hipMemcpy(ws + L.Jc, Jc, nc, hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(ws + L.kw, k_w, 8 * W, hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(ws + L.kref, kref_w, 8 * W, hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(ws + L.om, omega_w, 8 * W, hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(ws + L.kx2, kx2, 8 * (size_t)d->nx, hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(ws + L.ky2, ky2, 8 * (size_t)d->ny, hipMemcpyHostToDevice));
     UPPE_HIP(hipMemcpy(ws + L.filt, filt, 4 * W, hipMemcpyHostToDevice));
     UPPE_HIP(hipMemcpy(ws + L.damp, damp, 4 * lines, hipMemcpyHostToDevice));
     UPPE_HIP(hipMemcpy(ws + L.lx, lut_x, 4 * (size_t)d->lut_n, hipMemcpyHostToDevice));
@@ -644,6 +722,14 @@ int uppe_rf3d_plan_create(const uppe_rf3d_desc *d, void *workspace, size_t works
     p->et = (float *)(ws + L.et);
     p->enew = (float *)(ws + L.enew);
     p->e_scale = 1.0f;
+    p->ops.k = (const double *)(ws + L.kw);
+    p->ops.kref = (const double *)(ws + L.kref);
+    p->ops.om = (const double *)(ws + L.om);
+    p->ops.kx2 = (const double *)(ws + L.kx2);
+    p->ops.ky2 = (const double *)(ws + L.ky2);
+    p->ops.cpl = d->coupling;
+    p->ops.ny = d->ny;
+    p->ops.W = (int)W;
     RfPhys &ph = p->ph;
     // the reference passes these to its kernel as float32 and widens them to double inside
     ph.i_coef = (float)d->i_coef;
@@ -712,7 +798,7 @@ int uppe_rf3d_attempt(uppe_rf3d_plan *p, double h, void *stream, double *err) {
     xy_pass<1>(p, s, S, S, 0);
     Stage st;
     st.S = S;
-    st.D = (const float2 *)(ws + L.D);
+    st.ops = p->ops;
     st.nout = nout;
     st.v = (float2 *)(ws + L.v);
     st.a1 = (float2 *)(ws + L.a1);

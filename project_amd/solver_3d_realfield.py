@@ -5,7 +5,9 @@ Host side (FP64 NumPy, once per solver): grids, the propagator exponent
 D = -i (kz - k1 omega) with evanescent kz = -i sqrt|kz^2| (``solver_3d.py:54-89``), the P and J
 coupling coefficients -i omega^2 / (2 eps0 c^2 kz) and -omega / (2 eps0 c^2 kz) with the
 omega = 0 column zeroed (``:91-100``), the absorber (``:102-111``) and anti-alias filter
-(``:48-52``), cast to the reference's float32 / complex64 tables.  Device side (libuppe.so,
+(``:48-52``), cast to the reference's float32 / complex64 tables (the full D and coefficient
+arrays stay on the host for inspection; the device forms the same values per element from
+FP64 tables of k(omega), k1 omega, omega, kx^2 and ky^2).  Device side (libuppe.so,
 ``uppe_rf3d_*``): every attempt of the ERK4(3)-IP step with its low-order error estimate and
 the medium response (PPT table lookup, plasma density and Drude current in FP64, chi3 / chi5
 polarisation).  The accept / reject controller is the reference's (``:189-223``, exponent
@@ -43,6 +45,7 @@ class ERK43_RealField_UPPE_3D:
         self.dx, self.dy, self.dt = float(x[1] - x[0]), float(y[1] - y[0]), float(t[1] - t[0])
         kx = 2 * np.pi * np.fft.fftfreq(self.Nx, self.dx)
         ky = 2 * np.pi * np.fft.fftfreq(self.Ny, self.dy)
+        self.kx2, self.ky2 = kx ** 2, ky ** 2
         self.K2_perp_64 = kx[:, None] ** 2 + ky[None, :] ** 2
         self.omega_64 = 2 * np.pi * np.fft.rfftfreq(self.Nt, self.dt)
         self.freq_axis_hz = self.omega_64 / (2 * np.pi)
@@ -71,6 +74,7 @@ class ERK43_RealField_UPPE_3D:
         kp = float(np.asarray(self.medium.refractive_index(np.array([w0 + dw])))[0]) * (w0 + dw) / c
         self.k1 = (kp - k0) / dw
         kref = self.k1 * w
+        self.k_w, self.kref_w = k, kref
         kz2 = (k ** 2)[None, None, :] - self.K2_perp_64[:, :, None]
         kz = np.where(kz2 >= 0, np.sqrt(np.abs(kz2)).astype(np.complex128),
                       -1j * np.sqrt(np.abs(kz2)))
@@ -99,7 +103,8 @@ class ERK43_RealField_UPPE_3D:
             i_coef=0.5 * m.eps0 * m.c * m.n0, chi3_coef=m.eps0 * m.chi3,
             chi5_coef=m.eps0 * m.chi5 if m.chi5 != 0.0 else 0.0,
             av_coef=float(m.avalanche_coefficient()), rate_rec=1.0 / m.tau_rec,
-            rho_nt=m.rho_nt, Ui=m.Ui, e_charge=m.e_charge, m_e=m.m_e, tau_c=m.tau_c)
+            rho_nt=m.rho_nt, Ui=m.Ui, e_charge=m.e_charge, m_e=m.m_e, tau_c=m.tau_c,
+            coupling=2 * self.eps0 * float(self.c) ** 2)
 
     def _make_plan(self):
         import torch
@@ -109,9 +114,10 @@ class ERK43_RealField_UPPE_3D:
         _lib.check(lib.uppe_rf3d_workspace_size(ctypes.byref(d), ctypes.byref(nb)),
                    "uppe_rf3d_workspace_size")
         self._ws = torch.empty(nb.value, dtype=torch.uint8, device=self.device)
-        tabs = [np.ascontiguousarray(a) for a in (
-            self.D_operator, self.UPPE_P_coeff, self.UPPE_J_coeff, self.anti_alias_filter,
-            self.damp_space, self.medium.log10_I_f32, self.medium.W_f32)]
+        tabs = [np.ascontiguousarray(a, np.float64) for a in (
+            self.k_w, self.kref_w, self.omega_64, self.kx2, self.ky2)]
+        tabs += [np.ascontiguousarray(a, np.float32) for a in (
+            self.anti_alias_filter, self.damp_space, self.medium.log10_I_f32, self.medium.W_f32)]
         plan = ctypes.c_void_p()
         _lib.check(lib.uppe_rf3d_plan_create(
             ctypes.byref(d), _lib.ptr(self._ws), nb, *[_lib.host_ptr(a) for a in tabs],
