@@ -481,6 +481,8 @@ __global__ __launch_bounds__(256) void stage_kernel(Stage s, size_t n) {
     }
 }
 
+__global__ void zero2_kernel(unsigned int *p) { p[threadIdx.x] = 0u; }
+
 __global__ void scale_kernel(const float *in, float *out, float s, size_t n) {
     for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
         out[i] = in[i] * s;
@@ -791,7 +793,7 @@ int uppe_rf3d_attempt(uppe_rf3d_plan *p, double h, void *stream, double *err) {
     const size_t lines = (size_t)p->d.nx * p->d.ny, n = lines * (p->d.nt / 2 + 1);
     float2 *S = (float2 *)(ws + L.S), *x = (float2 *)(ws + L.x), *nout = (float2 *)(ws + L.nout);
     unsigned int *maxb = (unsigned int *)(ws + L.maxb);
-    UPPE_HIP(hipMemsetAsync(maxb, 0, 8, s));
+    zero2_kernel<<<1, 2, 0, s>>>(maxb);   // a kernel, not a memset node, should this be captured
     // S = FFT2(rFFT_t(E))
     line_rfft(p, s, p->et, S, 1.0f);
     xy_pass<1>(p, s, S, S, 1);

@@ -244,6 +244,7 @@ class MPC_Final_Runner:
         return self.apply_loss(A_norm)
 
     def run(self, cycles=None, snapshots=True):
+        import torch
         cycles = self.cycles if cycles is None else cycles
         A_norm = self.generate_initial_field()
         self._log("\n[Run] MPC simulation, %d cycles..." % cycles)
@@ -253,9 +254,10 @@ class MPC_Final_Runner:
         for i in range(1, cycles + 1):
             t0 = time.time()
             A_norm = self.cycle(A_norm)
+            torch.cuda.synchronize()
+            elapsed = time.time() - t0   # the cycle itself, as sim.py:337 (not the diagnostics)
             s = self._held(A_norm)
             snap = _dg.snapshot(s, self.w0_waist)
-            elapsed = time.time() - t0
             flu = snap["fluence"]
             has_nan = bool(np.isnan(flu).any() or not np.isfinite(snap["energy_total"]))
             max_flu = float(np.nanmax(flu)) if flu.size else 0.0
