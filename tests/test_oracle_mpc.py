@@ -136,3 +136,23 @@ def test_runner_segments_fill_the_cavity():
     assert [m for _, m in seq] == ["air", "fs", "air", "fs", "air", "fs", "air", "fs", "air"]
     assert sum(d for d, _ in seq) == pytest.approx(212.5e-3, rel=1e-15)
     assert seq[-1][0] == pytest.approx(59.5e-3)
+
+
+def test_runner_snapshot_figure_and_npz(tmp_path):
+    """The six-panel figure and the npz of sim.py:245-302 from a snapshot dict (no GPU)."""
+    r = pmpc.MPC_Final_Runner.__new__(pmpc.MPC_Final_Runner)
+    for k, v in pmpc.MPC_Final_Runner.DEFAULTS.items():
+        setattr(r, k, v)
+    r.output_dir = str(tmp_path)
+    x, y, t = grid(32, 128, hw=1e-3, tw=80e-12)
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    A = np.exp(-(X ** 2 + Y ** 2) / (2e-4) ** 2)[:, :, None] * np.exp(-t ** 2 / 6e-12 ** 2)
+    freq = np.fft.fftshift(np.fft.fftfreq(len(t), t[1] - t[0]))
+    snap = od.snapshot(A.astype(complex), x, y, t, 3e8, 215e-6, 1064e-9, freq)
+    r._plot(3, "Flat", snap)
+    assert (tmp_path / "cycle_03_Flat_MULTI_DIAG.png").stat().st_size > 10000
+    from project_amd import diagnostics as pdg
+    pdg.save_snapshot_npz(str(tmp_path / "cycle_03_Flat_DIAG.npz"), snap, 3, "Flat")
+    d = np.load(tmp_path / "cycle_03_Flat_DIAG.npz")
+    assert int(d["cycle"]) == 3 and str(d["label"]) == "Flat"
+    assert d["energy_total"] == pytest.approx(snap["energy_total"])
