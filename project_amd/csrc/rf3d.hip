@@ -17,10 +17,12 @@
 //   err = max|E4 - E_low| / (max|E4| + 1e-20);  E' = exp(-damp h) irFFT_t(filter iFFT2(E4))
 // with L = exp(h/2 D).  The controller (accept / reject, solver_3d.py:189-223) runs on the host.
 //
-// Kernels (first version, not fused): radix-4 Stockham line transforms through LDS (one
-// workgroup per t line; tiles of 16 adjacent omega for the x and y transforms, so every HBM
-// row is 128 contiguous bytes), one thread per (x, y) line for the serial plasma / Drude
-// recurrence (FP64, as the reference kernel), and elementwise stage kernels.
+// Kernels: radix-4 Stockham line transforms through LDS (one workgroup per t line; tiles of
+// 16 adjacent omega for the x and y transforms, so every HBM row is 128 contiguous bytes); a
+// line kernel that follows the inverse t transform with the sample-local response and the
+// forward transform of P; the plasma density / Drude current recurrence (FP64, as the
+// reference kernel) as a wave-level affine scan per line, or one thread per line where the
+// recurrence is nonlinear; elementwise stage kernels.
 #include <math.h>
 #include <stddef.h>
 #include <stdint.h>
@@ -197,30 +199,25 @@ __device__ __forceinline__ void coupling_at(const Ops &o, double K2, int w, floa
     jc = make_float2((float)r, (float)i);
 }
 
-// N_w = Pc rFFT_t(P) + Jc rFFT_t(J) per (x, y) line (media_physics.py returns P, J; the
+// N_w = Pc rFFT_t(P) + Jc rFFT_t(J) per (x, y) line; rFFT_t(P) comes from field_kernel (media_physics.py returns P, J; the
 // coefficient arrays are indexed like the field, solver_3d.py:121-143).
 template <int NT>
-__global__ __launch_bounds__(LT) void source_kernel(const float *__restrict__ P,
+__global__ __launch_bounds__(LT) void source_kernel(const float2 *__restrict__ Pw,
                                                     const float *__restrict__ J, float2 *out,
                                                     Ops ops, const float2 *__restrict__ tw) {
     constexpr int W = NT / 2 + 1;
-    __shared__ float2 a[NT], b[NT], pw[W];
+    __shared__ float2 a[NT], b[NT];
     const int tid = threadIdx.x;
     const size_t lt = (size_t)blockIdx.x * NT, lw = (size_t)blockIdx.x * W;
-    for (int t = tid; t < NT; t += LT) a[t] = make_float2(P[lt + t], 0.0f);
-    __syncthreads();
-    const float2 *r = group_fft<NT, 1>(a, b, 0, 0, tid, LT, tw);
-    for (int k = tid; k < W; k += LT) pw[k] = r[k];
-    __syncthreads();
     for (int t = tid; t < NT; t += LT) a[t] = make_float2(J[lt + t], 0.0f);
     __syncthreads();
-    r = group_fft<NT, 1>(a, b, 0, 0, tid, LT, tw);
+    const float2 *r = group_fft<NT, 1>(a, b, 0, 0, tid, LT, tw);
     const int line = blockIdx.x;
     const double K2 = ops.kx2[line / ops.ny] + ops.ky2[line % ops.ny];
     for (int k = tid; k < W; k += LT) {
         float2 pc, jc;
         coupling_at(ops, K2, k, pc, jc);
-        out[lw + k] = cadd(cmul(pc, pw[k]), cmul(jc, r[k]));
+        out[lw + k] = cadd(cmul(pc, Pw[lw + k]), cmul(jc, r[k]));
     }
 }
 
@@ -271,31 +268,54 @@ __device__ __forceinline__ float lut_interp(float lg, const float *lx, const flo
     return (float)(y0 + ((double)lg - x0) * (y1 - y0) / (x1 - x0));
 }
 
-// media_physics.py:258-318, the sample-local part: per sample (all in parallel) the
-// ionisation rate from the table and the polarisation P = eps0 (chi3 E^3 + chi5 E^5) / e_scale.
-// E: field lines (normalised), times e_scale (float32) = the physical field.
-__global__ __launch_bounds__(256) void rates_kernel(const float *__restrict__ En, float *P,
-                                                   float *Wi, RfPhys ph,
+// First half of a nonlinear evaluation (media_physics.py:258-318, the sample-local part), one
+// workgroup per (x, y) line: E = irFFT_t(Y) (C2R, as irfft_kernel; times scale, written for the
+// current pass); per sample the ionisation rate from the table (written) and the polarisation
+// P = eps0 (chi3 E^3 + chi5 E^5) / e_scale, with E times e_scale (float32) the physical field;
+// then rFFT_t(P), written for source_kernel.
+template <int NT>
+__global__ __launch_bounds__(LT) void field_kernel(const float2 *__restrict__ in, float *Eout,
+                                                   float *Wi, float2 *Pw, RfPhys ph,
                                                    const float *__restrict__ lx,
-                                                   const float *__restrict__ lw, size_t n,
-                                                   float in_scale) {
-    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * 256) {
-        const float Ef = En[i] * in_scale;
+                                                   const float *__restrict__ lw, float scale,
+                                                   float in_scale,
+                                                   const float2 *__restrict__ tw) {
+    __shared__ float2 a[NT], b[NT];
+    constexpr int W = NT / 2 + 1;
+    const int tid = threadIdx.x;
+    const float2 *src = in + (size_t)blockIdx.x * W;
+    for (int k = tid; k < W; k += LT) {
+        float2 v = src[k];
+        if (k == 0 || k == NT / 2) v.y = 0.0f;
+        a[k] = v;
+        if (k > 0 && k < NT / 2) a[NT - k] = make_float2(v.x, -v.y);
+    }
+    __syncthreads();
+    float2 *r = group_fft<NT, -1>(a, b, 0, 0, tid, LT, tw);
+    float2 *q = r == a ? b : a;   // free after the transform
+    const size_t lt = (size_t)blockIdx.x * NT;
+    for (int t = tid; t < NT; t += LT) {
+        const float e = r[t].x * scale;
+        Eout[lt + t] = e;
+        const float Ef = e * in_scale;
         const float I = ph.i_coef * (Ef * Ef);
-        Wi[i] = lut_interp(log10f(I + 1e-30f), lx, lw, ph);
+        Wi[lt + t] = lut_interp(log10f(I + 1e-30f), lx, lw, ph);
         float p = ((ph.chi3c * Ef) * Ef) * Ef;
         if (ph.use_chi5) {
             const float e2 = Ef * Ef;
             p = p + ((ph.chi5c * Ef) * e2) * e2;
         }
-        P[i] = p / ph.e_scale;
+        q[t] = make_float2(p / ph.e_scale, 0.0f);
     }
+    __syncthreads();
+    const float2 *rp = group_fft<NT, 1>(q, r, 0, 0, tid, LT, tw);
+    float2 *dst = Pw + (size_t)blockIdx.x * W;
+    for (int k = tid; k < W; k += LT) dst[k] = rp[k];
 }
 
 // The plasma kernel of media_physics.py:11-118 for one (x, y) line, serial in t (the density
 // with depletion and the Drude current are recurrences); FP64 as the reference.  Reads the
-// field and the rate of rates_kernel RCH samples ahead; writes J / e_scale.
+// field and the rate of field_kernel RCH samples ahead; writes J / e_scale.
 __device__ void current_serial(const float *__restrict__ En, const float *__restrict__ Wi,
                                float *J, const RfPhys &ph, size_t base, int nt, float in_scale) {
     double rho = 0.0, jf = 0.0;
@@ -494,7 +514,8 @@ __global__ void load_kernel(const float *in, float *out, float s, size_t n) {
 }
 
 struct Layout {
-    size_t et, enew, etmp, pb, jb, wib;            // real [lines][nt]
+    size_t et, enew, etmp, jb, wib;                // real [lines][nt]
+    size_t pw;                                     // complex [lines][W]: rFFT_t(P)
     size_t S, v, a1, low, x, nout, Y;              // complex [lines][W]
     size_t kw, kref, om, kx2, ky2;                 // FP64 operator tables
     size_t filt, damp, twt, twx, twy, lx, lw, maxb, total;
@@ -513,7 +534,7 @@ Layout layout(const uppe_rf3d_desc *d) {
     L.et = take(nr);
     L.enew = take(nr);
     L.etmp = take(nr);
-    L.pb = take(nr);
+    L.pw = take(nc);
     L.jb = take(nr);
     L.wib = take(nr);
     L.S = take(nc);
@@ -614,10 +635,29 @@ void line_irfft(uppe_rf3d_plan *p, hipStream_t s, const float2 *in, float *out,
 
 void line_source(uppe_rf3d_plan *p, hipStream_t s, float2 *out) {
     const int lines = p->d.nx * p->d.ny;
-    const float *P = (const float *)(p->ws + p->L.pb), *J = (const float *)(p->ws + p->L.jb);
+    const float2 *Pw = (const float2 *)(p->ws + p->L.pw);
+    const float *J = (const float *)(p->ws + p->L.jb);
     const float2 *tw = (const float2 *)(p->ws + p->L.twt);
     switch (p->d.nt) {
-#define C(NT) case NT: source_kernel<NT><<<lines, LT, 0, s>>>(P, J, out, p->ops, tw); break;
+#define C(NT) case NT: source_kernel<NT><<<lines, LT, 0, s>>>(Pw, J, out, p->ops, tw); break;
+        C(64) C(128) C(256) C(512) C(1024) C(2048)
+#undef C
+    }
+}
+
+// Y (x-space half spectra) -> E (etmp), rates (wib), rFFT_t(P) (pw)
+void line_field(uppe_rf3d_plan *p, hipStream_t s, const float2 *Y, float scale) {
+    const int lines = p->d.nx * p->d.ny;
+    float *E = (float *)(p->ws + p->L.etmp), *Wi = (float *)(p->ws + p->L.wib);
+    float2 *Pw = (float2 *)(p->ws + p->L.pw);
+    const float *lx = (const float *)(p->ws + p->L.lx), *lw = (const float *)(p->ws + p->L.lw);
+    const float2 *tw = (const float2 *)(p->ws + p->L.twt);
+    switch (p->d.nt) {
+#define C(NT)                                                                                 \
+    case NT:                                                                                 \
+        field_kernel<NT><<<lines, LT, 0, s>>>(Y, E, Wi, Pw, p->ph, lx, lw, scale, p->e_scale, \
+                                              tw);                                           \
+        break;
         C(64) C(128) C(256) C(512) C(1024) C(2048)
 #undef C
     }
@@ -649,12 +689,8 @@ void nonlinear(uppe_rf3d_plan *p, hipStream_t s, const float2 *X, float2 *out) {
     xy_pass<-1>(p, s, X, Y, 1);
     xy_pass<-1>(p, s, Y, Y, 0);
     const float inv = (float)(1.0 / ((double)p->d.nt * p->d.nx * p->d.ny));
-    line_irfft(p, s, Y, etmp, nullptr, nullptr, 0.0f, inv);
-    const size_t nr = (size_t)lines * p->d.nt;
+    line_field(p, s, Y, inv);
     float *wib = (float *)(p->ws + p->L.wib);
-    rates_kernel<<<stage_grid(nr), 256, 0, s>>>(etmp, (float *)(p->ws + p->L.pb), wib, p->ph,
-                                                 (const float *)(p->ws + p->L.lx),
-                                                 (const float *)(p->ws + p->L.lw), nr, p->e_scale);
     float *jb = (float *)(p->ws + p->L.jb);
     if (p->ph.use_av)
         current_kernel<<<(lines + 63) / 64, 64, 0, s>>>(etmp, wib, jb, p->ph, lines, p->d.nt,
