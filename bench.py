@@ -125,6 +125,7 @@ def bench_batched(n=256, L=1.0):
             "instances_done": done, "plan": [list(g_) for g_ in best.groups],
             "autotune": [[list(map(list, p)), round(t_, 4)] for (p, t_) in sw.tuning],
             "hbm_model_GBps": round(alg_bytes / secs / 1e9, 1),
+            "hbm_model_frac": round(alg_bytes / secs / 1e9 / HBM_PEAK_GBS, 4),
             "nt_counts": {str(k): int((nts == k).sum()) for k in sorted(set(nts))}}
 
 
