@@ -386,8 +386,11 @@ def main():
     solver.load(A0)
     s = _lib.stream_handle()
     _lib.check(lib.uppe3d_begin(solver._plan, 0.5, 1e-5, 1e-2, 1e-9, s), "uppe3d_begin")
-    for _ in range(args.warmup):
-        _lib.check(lib.uppe3d_attempt(solver._plan, s), "uppe3d_attempt")
+    # warm-up through the production driver (propagate's uppe3d_run, one status check at its
+    # end), which also sizes the plasma re-run grid from what it saw
+    stw = _lib.Uppe3DStatus()
+    _lib.check(lib.uppe3d_run(solver._plan, args.warmup, args.warmup, s, ctypes.byref(stw)),
+               "uppe3d_run")
     torch.cuda.synchronize()
     st0 = solver.status()
 

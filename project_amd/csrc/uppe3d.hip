@@ -68,6 +68,7 @@ struct Ctl3D {
     int plasma_groups[8];   // per t-pass slot of the attempt: workgroups deferred to plasma
     int adj_plasma;         // set by the adjoint t-pass when a line's plasma term matters
     int l2;                 // error norm: 0 max-norm ratio, 1 L2-norm ratio
+    int plasma_seen;        // workgroups deferred to plasma since uppe3d_begin
     double sumA, sumD;      // L2 norm: sum |A4|^2 and sum |FFT2(Y4 - Y5)|^2 of the attempt
 };
 
@@ -1824,7 +1825,10 @@ __global__ void a5pre_kernel(Adj3D a, Tabs3D tb, float h, int plane, size_t n) {
 // Step controller (solver_3d.py:182-208), one thread, FP64.  Also applies the top-of-loop
 // clip h = min(h, L - z) of the next iteration and the h < 1e-15 stop.
 __device__ void controller(Ctl3D *c) {
-    for (int i = 0; i < 8; ++i) c->plasma_groups[i] = 0;
+    for (int i = 0; i < 8; ++i) {
+        c->plasma_seen += c->plasma_groups[i];
+        c->plasma_groups[i] = 0;
+    }
     if (c->done) return;
     const double h = c->h;
     const unsigned int ab = c->maxA_bits, db = c->maxD_bits;
@@ -1896,6 +1900,7 @@ __global__ __launch_bounds__(1024) void ctl_phase_kernel(Ctl3D *c, Tabs3D tb, in
 
 __global__ void begin_kernel(Ctl3D *c, double L, double tol, double max_step, double min_step) {
     for (int i = 0; i < 8; ++i) c->plasma_groups[i] = 0;
+    c->plasma_seen = 0;
     c->z = 0.0;
     c->err = 0.0;
     c->replay_h = 0.0;
@@ -1941,6 +1946,7 @@ struct uppe3d_plan {
     Ctl3D *ctl;
     int *list;    // t-pass workgroups deferred to the plasma instantiation
     int ncu;      // compute units of the device (grid of the plasma launch)
+    int plasma_small;   // 1: the last uppe3d_run window deferred nothing to the plasma launch
     size_t ws_bytes;
     void *rec[3]; // optional copies of the stage fields X2, X3, X4 of every attempt
 };
@@ -2036,7 +2042,10 @@ int launch_tpass(uppe3d_plan *p, int mode, int slot, const float2 *in, int in_is
     const int nx = p->d.nx, ny = p->d.ny;
     const int lines = p->d.nt >= 2048 ? 8 : 16;
     const int groups = nx * (ny / lines);
-    const dim3 grid(groups), pgrid(groups < p->ncu ? groups : p->ncu);
+    // the plasma launch loops over its work list with a grid stride, so any grid is correct;
+    // while uppe3d_run's checks see an empty list a small grid saves launch time
+    const int pg = p->plasma_small ? 8 : p->ncu;
+    const dim3 grid(groups), pgrid(groups < pg ? groups : pg);
     switch (p->d.nt) {
         LAUNCH_T(128, 16)
         LAUNCH_T(256, 16)
@@ -2281,6 +2290,7 @@ int uppe3d_plan_create(const uppe3d_desc *d, void *workspace, size_t workspace_b
     p->ctl = (Ctl3D *)(ws + L.ctl);
     p->list = (int *)(ws + L.list);
     p->rec[0] = p->rec[1] = p->rec[2] = nullptr;
+    p->plasma_small = 0;
     {
         int dev = 0;
         UPPE_HIP(hipGetDevice(&dev));
@@ -2432,6 +2442,7 @@ int uppe3d_begin(uppe3d_plan *p, double L, double tol, double max_step, double m
     begin_kernel<<<1, 1, 0, (hipStream_t)stream>>>(p->ctl, L, tol, max_step, min_step);
     phase_kernel<<<(p->d.nt + 255) / 256, 256, 0, (hipStream_t)stream>>>(p->ctl, p->tb, p->d.nt);
     UPPE_HIP(hipGetLastError());
+    p->plasma_small = 0;   // full plasma grid until a run window has seen an empty work list
     return UPPE_OK;
 }
 
@@ -2481,11 +2492,16 @@ int uppe3d_replay_attempt(uppe3d_plan *p, double h, int accept, void *stream) {
     return enqueue_attempt(p, s);
 }
 
+static int read_ctl(uppe3d_plan *p, void *stream, Ctl3D *c) {
+    UPPE_HIP(hipMemcpyAsync(c, p->ctl, sizeof *c, hipMemcpyDeviceToHost, (hipStream_t)stream));
+    UPPE_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return UPPE_OK;
+}
+
 int uppe3d_status_get(uppe3d_plan *p, void *stream, uppe3d_status *st) {
     UPPE_CHECK(p && st, "uppe3d_status_get: null argument");
     Ctl3D c;
-    UPPE_HIP(hipMemcpyAsync(&c, p->ctl, sizeof c, hipMemcpyDeviceToHost, (hipStream_t)stream));
-    UPPE_HIP(hipStreamSynchronize((hipStream_t)stream));
+    RET(read_ctl(p, stream, &c));
     st->z = c.z;
     st->h = c.h;
     st->err = c.err;
@@ -2504,12 +2520,17 @@ int uppe3d_run(uppe3d_plan *p, int max_attempts, int check_every, void *stream,
     UPPE_CHECK(p && st, "uppe3d_run: null argument");
     if (check_every < 1) check_every = 1;
     int launched = 0;
+    Ctl3D c;
+    RET(read_ctl(p, stream, &c));
     RET(uppe3d_status_get(p, stream, st));
     while (!st->done && (max_attempts <= 0 || launched < max_attempts)) {
         int n = check_every;
         if (max_attempts > 0 && launched + n > max_attempts) n = max_attempts - launched;
+        const int seen = c.plasma_seen;
         for (int i = 0; i < n; ++i) RET(enqueue_attempt(p, (hipStream_t)stream));
         launched += n;
+        RET(read_ctl(p, stream, &c));
+        p->plasma_small = c.plasma_seen == seen ? 1 : 0;
         RET(uppe3d_status_get(p, stream, st));
     }
     if (st->nonfinite) {
