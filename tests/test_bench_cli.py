@@ -24,3 +24,13 @@ def test_help_lists_the_side_legs():
     assert r.returncode == 0
     for leg in ("mpc_3d", "realfield_3d", "cavity_1d", "batched_1d", "adjoint_3d"):
         assert leg in r.stdout
+
+
+def test_measured_traffic_is_read_from_the_profile():
+    """The bench line's roofline.traffic comes from the committed counter summary."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(ROOT, "bench.py"))
+    b = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(b)
+    t = b.measured_traffic(128, 1024)
+    assert t is not None and 2.6e8 < t < 2.8e8   # ~2 complex64 fields of 128 x 128 x 1024
