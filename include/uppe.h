@@ -92,7 +92,10 @@ int uppe3d_attempt_timed(uppe3d_plan *p, int n, void *stream, float *kernel_ms,
                          float *total_ms);
 /* One attempt with a prescribed step h and accept decision (step-sequence replay). */
 int uppe3d_replay_attempt(uppe3d_plan *p, double h, int accept, void *stream);
-/* Runs attempts until done or max_attempts, reading the status every `check_every` attempts. */
+/* Runs attempts until done or max_attempts, reading the status every `check_every` attempts.
+ * A window of attempts that deferred no t-pass workgroup to the plasma launch makes the
+ * following attempts of the plan launch it on a small grid (it loops over its work list, so the
+ * result is the same; uppe3d_begin restores the full grid). */
 int uppe3d_run(uppe3d_plan *p, int max_attempts, int check_every, void *stream,
                uppe3d_status *st);
 /* Synchronises `stream` and copies the control block. */
