@@ -686,6 +686,15 @@ __device__ __forceinline__ void tpass_group(int blk, float2 *stage, Fields3D f,
 // workgroups the first launch deferred (ctl->plasma_groups[slot] entries of list), a fixed
 // grid looping over them.  The counters are cleared by the controller at the end of each
 // attempt and by begin_kernel.
+// Workgroups of 8 lines (N_t = 2048) cover half of a 128-byte row of the state.  Blocks are
+// dealt round-robin to the 8 XCDs, each with its own L2, so the logical groups 2k and 2k + 1 (the
+// two halves of one row) are given to blocks g and g + 8: same XCD, dispatched together.  A
+// bijection of [0, 16 floor(n / 16)); the tail keeps its index.
+__device__ __forceinline__ int xcd_pair_group(int g, int n) {
+    if (g >= (n & ~15)) return g;
+    return 16 * (g >> 4) + 2 * (g & 7) + ((g >> 3) & 1);
+}
+
 template <int NT, int LINES, int MODE, bool PLASMA>
 __global__ __launch_bounds__(64 * LINES) void tpass_kernel(Fields3D f, const float2 *in_fixed,
                                                            int in_is_state, float2 *out,
@@ -694,7 +703,8 @@ __global__ __launch_bounds__(64 * LINES) void tpass_kernel(Fields3D f, const flo
     __shared__ float2 stage[TStage<NT, LINES>::lds(PLASMA)];
     if (ctl->done) return;
     if (!PLASMA) {
-        tpass_group<NT, LINES, MODE, false>(blockIdx.x, stage, f, in_fixed, in_is_state, out, tb,
+        const int g = LINES == 8 ? xcd_pair_group(blockIdx.x, gridDim.x) : blockIdx.x;
+        tpass_group<NT, LINES, MODE, false>(g, stage, f, in_fixed, in_is_state, out, tb,
                                             ph, ctl, nx, ny, list, slot);
     } else {
         const int n = ctl->plasma_groups[slot];
