@@ -344,7 +344,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--nx", type=int, default=128)
     ap.add_argument("--nt", type=int, default=1024)
-    ap.add_argument("--cpu-attempts", type=int, default=1,
+    ap.add_argument("--cpu-attempts", type=int, default=3,
                     help="oracle attempts timed for the CPU baseline (0 skips it)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the 1D, batched and adjoint legs")
