@@ -92,3 +92,38 @@ def test_c1_grid_replay(gpu):
     e = rel_l2(Aw_g, Aw_o)
     print("C1 rel L2", e)
     assert e < 1e-5
+
+
+def test_plasma_work_on_the_small_grid(gpu):
+    """After a run window that deferred no t-pass workgroup, the plan launches its plasma pass
+    on a small grid that loops over the work list.  Put a field with plasma on every central
+    line into that plan (uppe3d_set_state) and replay the oracle's steps: same result."""
+    import ctypes
+    import torch
+    from project_amd import _lib
+    # the "32x256 plasma scan" case (Kerr, no HOKE, 4 mJ on a 0.3 mm waist)
+    A_hi, oracle, solver = make(32, 256, 4e-3, n4=False, w0=0.3e-3, hw=1e-3)
+    x, y, t = solver.x, solver.y, solver.t
+    A_lo = wl.beam3d(x, y, t, energy=1e-9, w0=0.3e-3, tau_fwhm=40e-15)
+    _, _, Aw_o, log = oracle.propagate(A_hi, 0.005, tol=1e-5, max_attempts=6)
+    lib = _lib.load()
+    s = _lib.stream_handle()
+    solver.load(A_lo)
+    plan = solver._plan
+    _lib.check(lib.uppe3d_begin(plan, 1e300, 1.0, 1e-4, 0.0, s), "uppe3d_begin")
+    st = _lib.Uppe3DStatus()
+    announce("3d plasma on the small grid: plasma-free window")
+    _lib.check(lib.uppe3d_run(plan, 2, 2, s, ctypes.byref(st)), "uppe3d_run")
+    # the plasma field in this plan's units, layout [omega][x][y]
+    B = (np.fft.fft(A_hi, axis=2) / solver.A_scale).transpose(2, 0, 1)
+    Bd = torch.from_numpy(np.ascontiguousarray(B, np.complex64)).to(solver.device)
+    _lib.check(lib.uppe3d_set_state(plan, _lib.ptr(Bd), s), "uppe3d_set_state")
+    announce("3d plasma on the small grid: %d replayed attempts" % len(log))
+    for (h, _, acc) in log:
+        _lib.check(lib.uppe3d_replay_attempt(plan, float(h), int(bool(acc)), s),
+                   "uppe3d_replay_attempt")
+    solver.last_status = solver.status()
+    _, _, Aw_g = solver._finish(False)
+    e = rel_l2(Aw_g, Aw_o)
+    print("rel L2", e)
+    assert e < 2e-5
