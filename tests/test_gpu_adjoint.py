@@ -38,11 +38,14 @@ def oracle_solver(p_bar, sigma8=1e-130):
                                    plasma=pl))
 
 
-# config3 is kept off the GPU for now: one run of it faulted the GPU (illegal memory access
-# reported after the adjoint kernel) with an experimental build of the 1D kernels that has since
-# been taken out; no cause in this code has been shown.
+# config3 is kept off the GPU: one run of it faulted the GPU (illegal memory access reported
+# after the adjoint kernel) with an experimental build of the 1D kernels that has since been
+# taken out.  No cause has been shown: the current kernels run this exact case cleanly on the
+# CPU under AddressSanitizer with exact-size buffers (tests/test_emu_1d.py, which also checks
+# its gradients against the oracle), which clears the current code but names no cause.
 CONFIG3_SKIP = pytest.mark.skip(reason="GPU fault in one run with an experimental 1D build "
-                                       "(since removed); cause not shown, case kept off the GPU")
+                                       "(since removed); cause not shown, case kept off the GPU; "
+                                       "covered on the CPU by tests/test_emu_1d.py")
 
 
 @pytest.mark.parametrize("sigma8,L", [pytest.param(1e-130, 1.0, marks=CONFIG3_SKIP),
