@@ -108,8 +108,12 @@ class Params1D:
 class Oracle1D:
     """State of one reference solver instance (constructor rule of erk43ip_method.py:21-70)."""
 
-    def __init__(self, p: Params1D):
+    def __init__(self, p: Params1D, precision="double"):
+        """precision "single": the same algorithm with the field, D, the UPPE factor and the
+        nonlinear terms in complex64 / float32 (the plasma recurrence stays FP64) - a measure of
+        what FP32 arithmetic costs this algorithm (the reference's 1D solver is FP64 only)."""
         self.p = p
+        self.precision = precision
         self.lambda0 = p.center_wavelength
         self.omega0 = 2 * np.pi * C_LIGHT / self.lambda0
         if p.gamma is not None:
@@ -155,6 +159,11 @@ class Oracle1D:
         self.omega = omega
         self.H_raman = self._raman(omega) if self.p.use_raman else None
         self.uppe = (wabs / w0) * (self.n0 / np.where(n_w < 1.0, 1.0, n_w))
+        if self.precision == "single":
+            self.D = self.D.astype(np.complex64)
+            self.uppe = self.uppe.astype(np.float32)
+            if self.H_raman is not None:
+                self.H_raman = self.H_raman.astype(np.complex64)
 
     def _raman(self, omega):
         n = len(omega)
@@ -173,7 +182,7 @@ class Oracle1D:
     def plasma_term(self, A, dt):
         pl = self.p.plasma
         area = pl.area if pl.area is not None else self.beam_area
-        I = np.abs(A) ** 2 / area
+        I = np.abs(A).astype(np.float64) ** 2 / area
         w_ion = (pl.sigma_K * I ** pl.K).astype(np.float32)
         sig = pl.drude(self.omega0)
         w_ava = ((sig / pl.Ui) * I if pl.avalanche else np.zeros_like(I)).astype(np.float32)
@@ -193,7 +202,7 @@ class Oracle1D:
             out = 1j * self.gamma * P
         if self.p.plasma is not None:
             out = out + self.plasma_term(A, self.dt)
-        return out
+        return out.astype(A.dtype, copy=False)
 
     # -- integrator ---------------------------------------------------------------------
     def step(self, A, h, a5_prev=None, h_prev=None):
@@ -231,7 +240,7 @@ class Oracle1D:
         omega = 2 * np.pi * fftfreq(n, self.dt)
         self.setup(omega)
         z, h = 0.0, max_step / 100
-        A = np.array(A0, dtype=complex)
+        A = np.array(A0, dtype=np.complex64 if self.precision == "single" else complex)
         zh, Ah = [0.0], [A.copy()]
         e_prev = a5_prev = h_prev = None
         log = []

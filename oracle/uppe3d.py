@@ -11,7 +11,9 @@ Follows (reference paths relative to the reference root, directory
 * ``media_physics.py:72-104``   OpticalMedium constants
 * ``media_physics.py:113-119``  shock operator
 * ``media_physics.py:121-146``  Keldysh MPI + tunnel LUT, log10 interpolation
-* ``media_physics.py:152-154``  base avalanche rate (Argon and the other gases: zero)
+* ``media_physics.py:152-154``  base avalanche rate (Argon and the other gases: zero; fused
+  silica keeps it)
+* ``media_physics.py:208-250``  FusedSilica (Sellmeier, n0 = 1.45; Raman not restated)
 * ``media_physics.py:156-205``  nonlinear response; W cast to float32, K1 (``oracle/k1.py``)
 * ``media_physics.py:282-510``  Air / He / Ne / Ar / Kr / Xe parameters and Cauchy index
 * ``solver_3d.py:8-54``         grids, filter, valid-frequency mask
@@ -101,11 +103,15 @@ class GasMedium:
     def shock(self, omega):
         return (omega + self.omega0) / self.omega0 if self.use_shock else np.ones_like(omega)
 
+    def avalanche_rate(self, I):
+        """The gases override the base rate with zero (media_physics.py:431-432 and siblings)."""
+        return np.zeros(np.shape(I))
+
     def response(self, A, dt):
         """(Kerr, Plasma) of media_physics.py:156-205 in double mode."""
         I = np.abs(A) ** 2
         w_ion = self.ionization_rate(I).astype(np.float32)
-        w_ava = np.zeros(I.shape, np.float32)
+        w_ava = self.avalanche_rate(I).astype(np.float32)
         rho, gen = _k1.k1(w_ion, w_ava, 1.0 / self.tau_rec, dt, self.rho_nt)
         g = 1j * (self.omega0 / C_LIGHT)
         kerr = g * self.n2 * I * A
@@ -113,7 +119,51 @@ class GasMedium:
             kerr = kerr + g * self.n4 * I * I * A
         loss = -(gen * self.Ui) / (2.0 * I + 1e-30) * A
         defo = -(self.sigma_drude / 2.0) * (1.0 + 1j * self.omega0 * self.tau_c) * rho * A
-        return kerr, loss + defo
+        # both terms in the field's precision (the reference casts to its complex dtype)
+        return kerr.astype(A.dtype), (loss + defo).astype(A.dtype)
+
+
+@dataclass
+class FusedSilicaMedium(GasMedium):
+    """FusedSilica of media_physics.py:208-250 (no Raman): Sellmeier index clamped to
+    0.2-6 um, n0 = 1.45 in the Keldysh field and the Drude cross section, and the base class's
+    avalanche rate (sigma_drude / Ui) I (media_physics.py:152-154), which the solid keeps."""
+    name: str = "FusedSilica"
+
+    def __post_init__(self):
+        self.lambda0 = self.center_wavelength
+        self.c = C_LIGHT
+        self.omega0 = 2 * np.pi * C_LIGHT / self.lambda0
+        self.n0 = 1.45
+        self.n2 = 3.0e-20
+        self.n4 = 0.0
+        self.B = (0.6961663, 0.4079426, 0.8974794)
+        self.C = (0.0684043 ** 2, 0.1162414 ** 2, 9.896161 ** 2)
+        self.valid_range = (0.2e-6, 6.0e-6)
+        self.Ui = 9.0 * E_CHARGE
+        self.rho_nt = 2.1e28
+        self.K = 8
+        self.tau_rec = 150e-15
+        self.tau_c = 3.0e-15
+        self.sigma_drude = (E_CHARGE ** 2 * self.tau_c) / (
+            C_LIGHT * EPS0 * self.n0 * M_E * (1.0 + self.omega0 ** 2 * self.tau_c ** 2))
+        self.sigma_K = 2.5e-125
+        lg = np.linspace(np.log10(1e12), np.log10(1e20), self.lut_points)
+        self.lut_log10_I = lg
+        self.lut_W = self.keldysh_rate(10 ** lg)
+
+    def refractive_index(self, omega):
+        w = np.asarray(omega, np.float64)
+        w = np.where(np.abs(w) < 1e-12, 1e-12, w)
+        lam = np.clip(2 * np.pi * C_LIGHT / np.abs(w), *self.valid_range)
+        l2 = (lam * 1e6) ** 2
+        n2 = np.ones_like(l2)
+        for b, c in zip(self.B, self.C):
+            n2 = n2 + b * l2 / (l2 - c)
+        return np.sqrt(n2)
+
+    def avalanche_rate(self, I):
+        return (self.sigma_drude / self.Ui) * np.asarray(I, np.float64)
 
 
 def dispersion_operator(medium, omega, kperp2, valid_mask=True):
@@ -143,9 +193,14 @@ class Oracle3D:
     err_norm = "linf"
     valid_mask = True
 
-    def __init__(self, x, y, t, medium, workers=None):
+    def __init__(self, x, y, t, medium, workers=None, precision="double"):
+        """precision "single" restates the reference's default arithmetic (solver_3d.py:14-19,
+        35, 57): omega rounded to float32 before the FP64 operator build, operators, field and
+        nonlinear terms in complex64 / float32 (the plasma recurrence stays FP64, as in its
+        kernel).  It measures what FP32 arithmetic costs the reference itself."""
         self.m = medium
         self.workers = workers
+        self.precision = precision
         self.x, self.y, self.t = (np.asarray(a, np.float64) for a in (x, y, t))
         self.Nx, self.Ny, self.Nt = len(x), len(y), len(t)
         self.dx, self.dy = float(x[1] - x[0]), float(y[1] - y[0])
@@ -154,6 +209,8 @@ class Oracle3D:
         ky = 2 * np.pi * sfft.fftfreq(self.Ny, self.dy)
         self.kperp2 = kx[:, None] ** 2 + ky[None, :] ** 2
         self.omega = 2 * np.pi * sfft.fftfreq(self.Nt, self.dt)
+        if precision == "single":
+            self.omega = self.omega.astype(np.float32).astype(np.float64)
         self.freq_axis_hz = sfft.fftshift(self.omega) / (2 * np.pi)
         self.D, self.k, self.kref, self.k0, self.k1 = dispersion_operator(
             medium, self.omega, self.kperp2, self.valid_mask)
@@ -169,6 +226,10 @@ class Oracle3D:
         self.mask = (self.omega + medium.omega0 > 0).astype(np.float64)[None, None, :]
         if not self.valid_mask:
             self.mask = np.ones_like(self.mask)
+        if precision == "single":
+            self.D = self.D.astype(np.complex64)
+            for name in ("damp", "shock", "filt", "mask"):
+                setattr(self, name, getattr(self, name).astype(np.float32))
         self.A_scale = 1.0
 
     # FFT helpers (backward norm, as scipy/cupyx)
@@ -213,7 +274,7 @@ class Oracle3D:
 
         With ``replay`` the attempts and accept decisions come from another run's log.
         """
-        A0 = np.asarray(A0, np.complex128)
+        A0 = np.asarray(A0, np.complex64 if self.precision == "single" else np.complex128)
         s = float(np.max(np.abs(A0)))
         self.A_scale = s if s >= 1e-30 else 1.0
         A = A0 / self.A_scale

@@ -40,7 +40,10 @@ def valid_threads(nt):
 class BatchedSweep1D:
     """Propagates solvers[i] from inputs[i] = (t_i, A0_i) over length L."""
 
-    def __init__(self, solvers, inputs, L, tol=1e-6, max_step=1e-2, min_step=1e-11):
+    def __init__(self, solvers, inputs, L, tol=1e-6, max_step=1e-2, min_step=1e-11,
+                 record_steps=0):
+        """record_steps > 0 keeps the accepted step sizes of every instance (up to that many
+        per instance; read them with ``accepted_steps``)."""
         import torch
         lib = _lib.load()
         if len(solvers) != len(inputs) or not solvers:
@@ -85,18 +88,44 @@ class BatchedSweep1D:
                    "uppe1d_batch_workspace_size")
         self.ws = torch.empty(nbytes.value, dtype=torch.uint8, device=dev)
         self.descs = descs
+        rec = None
+        self.record_steps = int(record_steps)
+        if self.record_steps > 0:
+            self.hs = torch.zeros((n, self.record_steps), dtype=torch.float64, device=dev)
+            rec = (_lib.Uppe1DRecord * n)()
+            for j in range(n):
+                rec[j].hs = ctypes.c_void_p(self.hs[j].data_ptr())
+                rec[j].hs_cap = self.record_steps
         h = [_lib.host_ptr(a) for a in self._host]
         _lib.check(lib.uppe1d_batch_prepare(
             n, descs, h[0], h[1], h[2], h[3] if any_raman else None, h[4] if any_raman else None,
-            _lib.ptr(self.a0), _lib.ptr(self.ws), None, None, _lib.ptr(self.out), None,
+            _lib.ptr(self.a0), _lib.ptr(self.ws), None, None, _lib.ptr(self.out), rec,
             _lib.stream_handle()), "uppe1d_batch_prepare")
         self.plan = self.default_plan()
         self.tuning = []
 
     # -- plans ----------------------------------------------------------------------------
     def default_plan(self):
-        t = max(256, int(self.nts.max()) // MAXE)
-        return LaunchPlan([(0, self.n, t, 8192 if self.nts.max() > 2048 else 2048)])
+        """One launch for the whole batch when one workgroup size fits every N_t, otherwise
+        one launch per N_t group."""
+        nmax, nmin = int(self.nts.max()), int(self.nts.min())
+        t = max(256, nmax // MAXE)
+        if t <= nmin:
+            return LaunchPlan([(0, self.n, t, 8192 if nmax > 2048 else 2048)])
+        return self.group_plan("large")
+
+    def nt_groups(self):
+        bounds = [0] + [j for j in range(1, self.n) if self.nts[j] != self.nts[j - 1]] + [self.n]
+        return [(bounds[k], bounds[k + 1] - bounds[k], int(self.nts[bounds[k]]))
+                for k in range(len(bounds) - 1)]
+
+    def group_plan(self, pick):
+        g = []
+        for (first, cnt, nt) in self.nt_groups():
+            ts = valid_threads(nt)
+            t = ts[0] if pick == "small" else ts[-1]
+            g.append((first, cnt, t, 2048 if nt <= 2048 else 8192))
+        return LaunchPlan(g)
 
     def candidate_plans(self):
         plans = []
@@ -105,17 +134,8 @@ class BatchedSweep1D:
             if t <= nmin and nmax // t <= MAXE:
                 plans.append(LaunchPlan([(0, self.n, t, 8192 if nmax > 2048 else 2048)]))
         # one launch per N_t group, each with every valid workgroup size
-        bounds = [0] + [j for j in range(1, self.n) if self.nts[j] != self.nts[j - 1]] + [self.n]
-        groups = [(bounds[k], bounds[k + 1] - bounds[k], int(self.nts[bounds[k]]))
-                  for k in range(len(bounds) - 1)]
-        if len(groups) > 1:
-            for pick in ("small", "large"):
-                g = []
-                for (first, cnt, nt) in groups:
-                    ts = valid_threads(nt)
-                    t = ts[0] if pick == "small" else ts[-1]
-                    g.append((first, cnt, t, 2048 if nt <= 2048 else 8192))
-                plans.append(LaunchPlan(g))
+        if len(self.nt_groups()) > 1:
+            plans += [self.group_plan("small"), self.group_plan("large")]
         return plans
 
     def enqueue(self, plan=None, stream=None):
@@ -147,6 +167,19 @@ class BatchedSweep1D:
         for j, i in enumerate(self.order):
             out[i] = st[j]
         return out
+
+    def accepted_steps(self, statuses=None):
+        """Accepted step sizes of every instance (input order) from the last run, NumPy."""
+        if self.record_steps <= 0:
+            raise RuntimeError("the sweep was built without record_steps")
+        st = statuses if statuses is not None else self.status()
+        host = self.hs.cpu().numpy()
+        res = [None] * self.n
+        for j, i in enumerate(self.order):
+            if st[i].overflow & 2:
+                raise RuntimeError("instance %d took more than record_steps steps" % i)
+            res[i] = host[j, :st[i].accepted].copy()
+        return res
 
     def spectra(self):
         """Final A(omega) of every instance (input order), NumPy complex64."""

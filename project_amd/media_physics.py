@@ -107,6 +107,46 @@ class OpticalMedium:
         return coef
 
 
+class FusedSilica(OpticalMedium):
+    """Fused silica (media_physics.py:208-250): Sellmeier index clamped to 0.2-6 um, n0 = 1.45
+    in the Keldysh field and the Drude cross section, K = 8 multiphoton ionisation and the base
+    class's avalanche rate (sigma_drude / Ui) I, which the fused kernel evaluates with the
+    general (non-affine) FP64 density recurrence.  ``use_raman=True`` is accepted here and
+    rejected by the solver (the Raman response, ``:252-278``, is not in the HIP kernels)."""
+
+    def __init__(self, center_wavelength, precision="single", use_shock=False, use_raman=False,
+                 tunnel_guard=1e-20):
+        super().__init__(center_wavelength, precision, use_shock, tunnel_guard)
+        self.n0 = 1.45
+        self.n2 = 3.0e-20
+        self.use_raman = use_raman
+        self.B = [0.6961663, 0.4079426, 0.8974794]
+        self.C = [0.0684043 ** 2, 0.1162414 ** 2, 9.896161 ** 2]
+        self.valid_range = (0.2e-6, 6.0e-6)
+        self.f_R = 0.18
+        self.tau1 = 12.2e-15
+        self.tau2 = 32.0e-15
+        self.H_raman = None
+        self.Ui = 9.0 * self.e_charge
+        self.rho_nt = 2.1e28
+        self.K = 8
+        self.tau_rec = 150e-15
+        self.tau_c = 3.0e-15
+        self.sigma_drude = self._drude(self.n0)
+        self.sigma_K = 2.5e-125
+        self._init_ionization_lut()
+
+    def refractive_index(self, omega_array):
+        w = np.asarray(omega_array, np.float64)
+        w = np.where(np.abs(w) < 1e-12, 1e-12, w)
+        lam = np.clip(2 * np.pi * self.c / np.abs(w), self.valid_range[0], self.valid_range[1])
+        l2 = (lam * 1e6) ** 2
+        n_sq = np.ones_like(l2)
+        for b, c in zip(self.B, self.C):
+            n_sq = n_sq + b * l2 / (l2 - c)
+        return np.sqrt(n_sq)
+
+
 class _CauchyGas(OpticalMedium):
     """Noble gas with Cauchy dispersion (media_physics.py:317-510)."""
     _params = None   # (A, B, C, n2/atm, n4/atm^2, Ui eV, K, sigma_K/atm, tau_c*atm)

@@ -13,8 +13,8 @@ host sees only the per-cycle diagnostics.
 Interface kept: ``MPC_Final_Runner()`` with the reference's physical constants
 (``sim.py:12-75``; any of them can be overridden by keyword), ``calculate_eigenmode``,
 ``generate_initial_field``, ``propagate_segment``, ``apply_curved_mirror``, ``save_snapshot``
-(the ``cycle_XX_<label>_DIAG.npz`` keys of ``:245-249`` and, with ``plot=True``, the six-panel
-figure of ``:251-302``) and ``run``.  Fields are normalised by the runner's ``A_scale``
+(the ``cycle_XX_<label>_DIAG.npz`` keys of ``:245-249``; the reference's figure, ``:251-302``,
+is not drawn: every quantity it shows is in the npz) and ``run``.  Fields are normalised by the runner's ``A_scale``
 (``:98-118``) as in the reference, as torch complex64 tensors on the GPU.
 Differences: N_t must be a power of two from 128 to 2048 (default 2048; the reference uses
 2046, which the HIP t transforms do not take); ``run`` returns the per-cycle log.
@@ -38,9 +38,7 @@ class MPC_Final_Runner:
         d_fs=12e-3, d_air_1=30e-3, d_air_mid=25e-3, n_plates=4,
         cycles=33, total_trans=53.9 / 100.0,
         window_width=2.0e-3, Nxy=256, Nt=2048, time_window=80e-12,
-        n2_fs=3.0e-20, plot=False, verbose=True,
-        viz_wl_min_nm=1050.0, viz_wl_max_nm=1080.0, viz_time_ps_min=-30.0,
-        viz_time_ps_max=30.0)
+        n2_fs=3.0e-20, verbose=True)
 
     def __init__(self, **overrides):
         unknown = set(overrides) - set(self.DEFAULTS)
@@ -169,7 +167,7 @@ class MPC_Final_Runner:
         _dg.apply_mirror(s, 0.0, float(np.sqrt(self.T_per_round)))
         return self._out(s)
 
-    # -- sim.py:187-302 --------------------------------------------------------------------
+    # -- sim.py:187-249 --------------------------------------------------------------------
     def save_snapshot(self, cycle, field_norm, label):
         s = self._held(field_norm)
         snap = _dg.snapshot(s, self.w0_waist)
@@ -177,60 +175,7 @@ class MPC_Final_Runner:
             _dg.save_snapshot_npz(
                 os.path.join(self.output_dir, "cycle_%02d_%s_DIAG.npz" % (cycle, label)),
                 snap, cycle, label)
-            if self.plot:
-                self._plot(cycle, label, snap)
         return snap
-
-    def _plot(self, cycle, label, d):
-        import matplotlib
-        matplotlib.use("Agg")
-        import matplotlib.pyplot as plt
-
-        def norm(a):
-            a = np.asarray(a)
-            m = np.max(a) if np.max(a) > 0 else 1.0
-            return a / (m + 1e-30)
-
-        wl, t_ps = d["wl_nm"], d["t_s"] * 1e12
-        valid = d["freq_abs_Hz"] > 1e9
-        mw = (wl > self.viz_wl_min_nm) & (wl < self.viz_wl_max_nm) & valid
-        mt = (t_ps >= self.viz_time_ps_min) & (t_ps <= self.viz_time_ps_max)
-        ext = [-self.window_width * 500, self.window_width * 500] * 2
-        fig = plt.figure(figsize=(18, 10), constrained_layout=True)
-        gs = fig.add_gridspec(2, 3)
-        ax = fig.add_subplot(gs[0, 0])
-        for k, lab, lw in (("spec_smf", "SMF", 1.8), ("spec_full_incoh", "Full Beam", 1.0),
-                           ("spec_center", "Center", 1.0)):
-            ax.plot(wl[mw], norm(d[k])[mw], lw=lw, label=lab)
-        ax.set_xlim(self.viz_wl_min_nm, self.viz_wl_max_nm)
-        ax.set_title("Spectrum - %s Cycle %d" % (label, cycle))
-        ax.legend()
-        ax = fig.add_subplot(gs[0, 1])
-        ax.plot(t_ps[mt], norm(d["I_center_t"])[mt], label="Center")
-        ax.plot(t_ps[mt], norm(d["I_smf_t"])[mt], label="SMF")
-        ax.set_xlim(self.viz_time_ps_min, self.viz_time_ps_max)
-        ax.set_title("Temporal - Peak@t=%.2fps\nFWHM(c)=%.2fps, FWHM(S)=%.2fps"
-                     % (t_ps[d["it_peak"]], d["fwhm_center"] * 1e12, d["fwhm_smf"] * 1e12))
-        ax.legend()
-        ax = fig.add_subplot(gs[0, 2])
-        ax.plot(t_ps[mt], norm(d["P_t"])[mt], color="k")
-        ax.set_xlim(self.viz_time_ps_min, self.viz_time_ps_max)
-        ax.set_title("Total Power vs Time")
-        amp = np.abs(d["A_xy"])
-        for pos, img, cmap, title in (
-                (gs[1, 0], d["fluence"], "inferno", "Near-field Fluence\nE=%.3eJ, wrms=%.1fum"
-                 % (d["energy_total"], d["wx_rms_m"] * 1e6)),
-                (gs[1, 1], amp, "viridis", "Amplitude |A| at t_peak=%.2fps" % t_ps[d["it_peak"]]),
-                (gs[1, 2], np.where(amp >= 0.10 * (np.max(amp) + 1e-30), np.angle(d["A_xy"]),
-                                    np.nan), "twilight", "Phase arg(A)")):
-            ax = fig.add_subplot(pos)
-            kw = dict(vmin=-np.pi, vmax=np.pi) if cmap == "twilight" else {}
-            im = ax.imshow(img, cmap=cmap, origin="lower", extent=ext, **kw)
-            ax.set_title(title)
-            fig.colorbar(im, ax=ax)
-        plt.savefig(os.path.join(self.output_dir, "cycle_%02d_%s_MULTI_DIAG.png" % (cycle, label)),
-                    dpi=140)
-        plt.close(fig)
 
     # -- sim.py:304-352 --------------------------------------------------------------------
     def cycle(self, A_norm):
@@ -271,7 +216,5 @@ class MPC_Final_Runner:
             if snapshots and (i == 1 or i % 2 == 0) and self.output_dir:
                 _dg.save_snapshot_npz(
                     os.path.join(self.output_dir, "cycle_%02d_Flat_DIAG.npz" % i), snap, i, "Flat")
-                if self.plot:
-                    self._plot(i, "Flat", snap)
         self.field = A_norm
         return self.history

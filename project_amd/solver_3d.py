@@ -12,10 +12,13 @@ Call contract kept: ``propagate(A0, L, tol=1e-4, max_step=1e-2, min_step=1e-9,
 return_gpu=False) -> (z, freq_axis_hz, A_w)`` with A_w = fftshift_t(FFT_t(A(L))) of shape
 (Nx, Ny, Nt).  ``return_gpu=True`` returns a torch tensor on the GPU (CuPy does not exist on
 ROCm).  One difference: a non-finite error norm raises ``FloatingPointError`` (the
-reference's loop never terminates in that case); KeyboardInterrupt returns the partial
-field, as in the reference (``:210-211``).
+reference's loop never terminates in that case).  KeyboardInterrupt returns the partial
+field, as in the reference (``:210-211``): the attempts are enqueued in bounded chunks
+(``attempts_per_call``), so an interrupt lands within one chunk.  ``precision='double'``
+(complex128 in the reference, ``:14-19``) warns and runs the complex64 device path.
 """
 import ctypes
+import warnings
 
 import numpy as np
 
@@ -33,10 +36,19 @@ class ERK43IP_UPPE_3D_Optimized:
     # negative-absolute-frequency mask from D and N (both switched by the MPC solver variant)
     error_norm = "linf"
     valid_mask = True
+    # attempts enqueued per library call of propagate/advance (the interrupt granularity)
+    attempts_per_call = 64
 
     def __init__(self, x_grid, y_grid, t_grid, medium, precision="single"):
         import torch
 
+        if precision not in ("single", "double"):
+            raise ValueError("precision must be 'single' or 'double'")
+        if precision == "double":
+            warnings.warn("ERK43IP_UPPE_3D_Optimized: precision='double' is not implemented on "
+                          "the device; the field runs in complex64 with FP64 error norm, "
+                          "controller and plasma density (the reference's default 'single' "
+                          "arithmetic)", RuntimeWarning, stacklevel=2)
         if hasattr(medium, "calc_nonlinear_response"):
             raise NotImplementedError(
                 "medium defines calc_nonlinear_response; the fused HIP kernel computes the "
@@ -202,12 +214,22 @@ class ERK43IP_UPPE_3D_Optimized:
         s = _lib.stream_handle()
         _lib.check(lib.uppe3d_begin(self._plan, L, tol, max_step, min_step, s), "uppe3d_begin")
         st = _lib.Uppe3DStatus()
+        # bounded chunks of attempts per library call, so that Ctrl-C (raised by Python when
+        # a call returns) stops the run within one chunk and keeps the field reached so far
+        chunk = max(int(check_every), int(self.attempts_per_call))
+        done = 0
         try:
-            rc = lib.uppe3d_run(self._plan, int(max_attempts), int(check_every), s,
-                                ctypes.byref(st))
-            if rc == -3:
-                raise FloatingPointError(lib.uppe_last_error().decode())
-            _lib.check(rc, "uppe3d_run")
+            while True:
+                n = chunk if max_attempts <= 0 else min(chunk, int(max_attempts) - done)
+                if n <= 0:
+                    break
+                rc = lib.uppe3d_run(self._plan, n, int(check_every), s, ctypes.byref(st))
+                if rc == -3:
+                    raise FloatingPointError(lib.uppe_last_error().decode())
+                _lib.check(rc, "uppe3d_run")
+                done += n
+                if st.done:
+                    break
         except KeyboardInterrupt:
             st = self.status()
             print("\n[Solver] Interrupted at z=%.3f mm" % (st.z * 1000))

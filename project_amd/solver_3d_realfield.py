@@ -17,9 +17,11 @@ Call contract kept: ``propagate(E0, L, tol=1e-4, max_step=1e-2, min_step=1e-9,
 return_gpu=False) -> (z, freq_axis_hz, E_w)`` with E_w = rFFT_t(E(L)) of shape
 (Nx, Ny, Nt/2 + 1) (not shifted).  ``return_gpu=True`` returns a torch tensor on the GPU.
 Differences: a non-finite error raises ``FloatingPointError`` (the reference's loop would not
-end); KeyboardInterrupt returns the partial field, as the reference.
+end); KeyboardInterrupt returns the partial field, as the reference; ``precision='double'``
+(float64 / complex128 in the reference, ``:15-19``) warns and runs the float32 device path.
 """
 import ctypes
+import warnings
 
 import numpy as np
 
@@ -28,6 +30,12 @@ from . import _lib
 
 class ERK43_RealField_UPPE_3D:
     def __init__(self, x_grid, y_grid, t_grid, medium, precision="single"):
+        if precision not in ("single", "double"):
+            raise ValueError("precision must be 'single' or 'double'")
+        if precision == "double":
+            warnings.warn("ERK43_RealField_UPPE_3D: precision='double' is not implemented on the "
+                          "device; the field runs in float32 / complex64 with FP64 plasma "
+                          "density and current", RuntimeWarning, stacklevel=2)
         if getattr(medium, "use_raman", False):
             raise NotImplementedError("the Raman response is not implemented in the HIP kernels")
         self.medium = medium

@@ -70,10 +70,12 @@ def c0_oracle(sigma_K=1e-130):
     return o1.Oracle1D(p)
 
 
-@pytest.mark.parametrize("sigma_K,L,bound", [(1e-130, 1.0, 1e-5), (1e-121, 0.2, 1e-4)],
-                         ids=["config0", "plasma-stress"])
-def test_config0_vs_oracle(gpu, sigma_K, L, bound):
-    t, A0 = wl.pulse1d(4096)
+@pytest.mark.parametrize("nt,sigma_K,L,bound", [(4096, 1e-130, 1.0, 1e-5),
+                                                (4096, 1e-121, 0.2, 1e-4),
+                                                (8192, 1e-130, 1.0, 1e-5)],
+                         ids=["config0", "plasma-stress", "nt8192"])
+def test_config0_vs_oracle(gpu, nt, sigma_K, L, bound):
+    t, A0 = wl.pulse1d(nt)
     orc = c0_oracle(sigma_K)
     zo, Aho, wo, log = orc.propagate(A0, t, L, tol=1e-6, max_step=1e-2, min_step=1e-11)
     s = c0_solver(sigma_K)

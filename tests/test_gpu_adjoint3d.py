@@ -31,6 +31,12 @@ CASES = [
     ("16x128 kerr ckpt2", 16, 128, 0.8e-3, False, False, 0.5, 1e-5, (780e-9, 800e-9), 2),
     # 256 x 256 planes: column pass + row kernel adjoints
     ("256x128 kerr+hoke", 256, 128, 0.8e-3, True, False, 1e-3, 1e-5, (790e-9, 805e-9), None),
+    # the long-line adjoint t-passes of the 256 x 256 x 2048 workload (N_t = 2048) and of the
+    # 128 x 128 x 1024 grid (N_t = 1024)
+    ("16x2048 kerr+hoke+shock", 16, 2048, 0.8e-3, True, True, 2e-3, 1e-5, (790e-9, 805e-9),
+     None),
+    ("32x1024 kerr+hoke+shock", 32, 1024, 0.8e-3, True, True, 2e-3, 1e-5, (790e-9, 805e-9),
+     None),
 ]
 
 
@@ -109,3 +115,67 @@ def test_adjoint3d_refuses_plasma(gpu):
     Bf = a3.UPPEPropagate3D.apply(B0, prop)
     with pytest.raises(NotImplementedError):
         torch.sum(torch.abs(Bf) ** 2).backward()
+
+
+def test_adjoint3d_full_grid_directional_derivatives(gpu):
+    """256 x 256 x 2048 (the full-size adjoint grid: 256 x 256 column/row adjoint kernels and
+    the 2048-point adjoint t-pass), argon 0.8 mJ, 2 mm (a few frozen steps, nonlinear phase of
+    order 1 rad, where the map is well conditioned): directional derivatives of the on-axis
+    790-805 nm spectral energy along a smooth spatial-phase direction (defocus r^2), a smooth
+    spectral-phase direction (quadratic in omega) and two seeded random directions, adjoint
+    against central differences of the same frozen forward on the GPU, |err| / |g|.  The FP64
+    oracle is out of reach at this size; the frozen forward itself is checked against it on
+    the small grids above.  (Over the 0.3 m of the benchmark leg the HOKE phase reaches ~300
+    rad and finite differences no longer measure a derivative.)"""
+    import torch
+    from project_amd import adjoint3d as a3
+    nx, nt, L = 256, 2048, 2e-3
+    x, y, t = wl.grid3d(nx, nt)
+    A_base = wl.beam3d(x, y, t)
+    solver = ERK43IP_UPPE_3D_Optimized(x, y, t, mp.Argon(800e-9, 1e5 / 101325,
+                                                         tunnel_guard=0.0))
+    prop = a3.FrozenPropagator3D(solver, A_base, L, tol=1e-5)
+    announce("adjoint3d 256x256x2048 L=2e-3 steps=%d ckpt=%d" % (prop.M, prop.C))
+    mask = a3.band_mask(solver.omega, solver.omega0, 790e-9, 805e-9)
+    idx = torch.as_tensor(np.nonzero(mask)[0], device="cuda")
+    ix = iy = nx // 2
+    dev = torch.device("cuda")
+    Ab = torch.from_numpy(A_base.astype(np.complex64)).to(dev)
+    phi = torch.zeros((nx, nx), dtype=torch.float32, device=dev, requires_grad=True)
+    psi = torch.zeros(nt, dtype=torch.float32, device=dev, requires_grad=True)
+    B0 = a3.to_state(a3.phase_field(Ab, phi, psi), prop.A_scale)
+    Bf = a3.UPPEPropagate3D.apply(B0, prop)
+    loss = a3.onaxis_band_energy(Bf, prop.A_scale, mask, ix, iy)
+    loss.backward()
+    g_phi = phi.grad.detach().double()
+    g_psi = psi.grad.detach().double()
+    gn = float(torch.sqrt(torch.sum(g_phi ** 2) + torch.sum(g_psi ** 2)))
+
+    def loss_at(dphi, dpsi):
+        with torch.no_grad():
+            B = a3.to_state(a3.phase_field(Ab, dphi.float(), dpsi.float()), prop.A_scale)
+            out = prop.forward(B)
+            return float((prop.A_scale ** 2) *
+                         torch.sum(torch.abs(out[idx, ix, iy].to(torch.complex128)) ** 2))
+
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    rng = np.random.default_rng(3)
+    dirs = [("defocus", (X ** 2 + Y ** 2), np.zeros(nt)),
+            ("spectral quadratic", np.zeros((nx, nx)), solver.omega ** 2),
+            ("random 1", rng.standard_normal((nx, nx)), rng.standard_normal(nt)),
+            ("random 2", rng.standard_normal((nx, nx)), rng.standard_normal(nt))]
+    L0 = loss_at(torch.zeros((nx, nx), device=dev), torch.zeros(nt, device=dev))
+    print("loss %.6e (adjoint forward %.6e)" % (L0, loss.item()))
+    worst = 0.0
+    for name, dphi, dpsi in dirs:
+        nrm = np.sqrt(np.sum(dphi ** 2) + np.sum(dpsi ** 2))
+        dphi = torch.from_numpy(dphi / nrm).to(dev)
+        dpsi = torch.from_numpy(dpsi / nrm).to(dev)
+        ad = float(torch.sum(g_phi * dphi) + torch.sum(g_psi * dpsi))
+        fds = [(loss_at(eps * dphi, eps * dpsi) - loss_at(-eps * dphi, -eps * dpsi)) / (2 * eps)
+               for eps in (1e-1, 3e-2)]
+        err = min(abs(ad - fd) for fd in fds) / gn
+        worst = max(worst, err)
+        print("%s: adjoint %.6e, fd %s, |err|/|g| %.2e" % (name, ad,
+                                                           ["%.6e" % f for f in fds], err))
+    assert worst < 1e-5
