@@ -257,6 +257,19 @@ int uppe1d_propagate(const uppe1d_desc *d, const double *D_re, const double *D_i
                      const void *a0, void *workspace, void *hist, void *zhist, void *out_w,
                      const uppe1d_record *rec, void *stream, uppe1d_status *st);
 
+/* Allocation-free variant of uppe1d_propagate (same arguments without `workspace`; same
+ * kernels and results).  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation).  The instance workspace lives in a static device array of the library
+ * (about 0.5 MB per device, held until the process ends), sized for nt up to 8192 (larger nt
+ * is rejected); the tables are staged in a static host array.  Calls on one device share that
+ * array: calls that overlap on one GPU race (host calls are serialised by a lock, but work
+ * enqueued on different streams is not).  Synchronises the stream. */
+int uppe1d_propagate_noalloc(const uppe1d_desc *d, const double *D_re, const double *D_im,
+                             const double *uppe_factor, const double *H_re, const double *H_im,
+                             const void *a0, void *hist, void *zhist, void *out_w,
+                             const uppe1d_record *rec, void *stream, uppe1d_status *st);
+
 /* Slice chain (cavity / multipass drivers; replaces the host loop of gas.py:113-147 and
  * "gas copy.py":115-141 that calls propagate once per slice with a new gamma).  All slices run
  * in order in one kernel launch, with no host round trip between them.  Slice i is a full

@@ -109,6 +109,9 @@ SIGNATURES = {
     "uppe1d_workspace_size": (c_int, [P(Uppe1DDesc), P(c_size_t)]),
     "uppe1d_propagate": (c_int, [P(Uppe1DDesc)] + [c_void_p] * 5
                          + [c_void_p] * 5 + [P(Uppe1DRecord), c_void_p, P(Uppe1DStatus)]),
+    "uppe1d_propagate_noalloc": (c_int, [P(Uppe1DDesc)] + [c_void_p] * 5
+                                 + [c_void_p] * 4 + [P(Uppe1DRecord), c_void_p,
+                                                     P(Uppe1DStatus)]),
     "uppe1d_slices_workspace_size": (c_int, [P(Uppe1DDesc), c_int, P(c_size_t)]),
     "uppe1d_propagate_slices": (c_int, [P(Uppe1DDesc), c_int, P(Uppe1DSlice)] + [c_void_p] * 5
                                 + [c_void_p] * 6 + [c_void_p, P(Uppe1DStatus)]),

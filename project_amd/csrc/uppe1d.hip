@@ -18,6 +18,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <mutex>
 #include <vector>
 
 #include "abi_common.hpp"
@@ -1291,14 +1292,16 @@ struct Header1D {
     int counters[30];   // one per concurrent launch slot
 };
 
-size_t align256(size_t v) { return (v + 255) / 256 * 256; }
+constexpr size_t align256(size_t v) { return (v + 255) / 256 * 256; }
 
-size_t inst_bytes(int nt) {
+constexpr size_t inst_bytes(int nt) {
     return align256(16 * (size_t)nt) + align256(4 * (size_t)nt) + 5 * align256(8 * (size_t)nt) +
            align256(sizeof(Stat1D));
 }
 
-size_t header_bytes(int n) { return align256(sizeof(Header1D)) + align256(sizeof(Inst1D) * n); }
+constexpr size_t header_bytes(int n) {
+    return align256(sizeof(Header1D)) + align256(sizeof(Inst1D) * n);
+}
 
 int check1d(const uppe1d_desc &d) {
     UPPE_CHECK(d.nt >= 256 && d.nt <= 8192 && (d.nt & (d.nt - 1)) == 0,
@@ -1308,6 +1311,82 @@ int check1d(const uppe1d_desc &d) {
     UPPE_CHECK(!d.use_plasma || (d.K >= 1 && d.area > 0.0 && d.rho_nt > 0.0 && d.tau_rec > 0.0),
                "uppe1d: plasma needs K >= 1, area > 0, rho_nt > 0, tau_rec > 0");
     return UPPE_OK;
+}
+
+// Scalars and tables of one instance: the tables are written to the host staging area hb in
+// the device layout at db (the instance's slice of the workspace); P gets the device
+// pointers.  Returns the bytes used (inst_bytes(d.nt)).  D_re .. H_im point at this
+// instance's nt values (H may be null).
+size_t fill_instance(const uppe1d_desc &d, const double *D_re, const double *D_im,
+                     const double *uppe_factor, const double *H_re, const double *H_im,
+                     char *hb, char *db, Inst1D &P) {
+    const int nt = d.nt;
+    memset(&P, 0, sizeof P);
+    P.nt = nt;
+    P.max_hist = d.max_hist > 0 ? d.max_hist : 0;
+    P.max_attempts = d.max_attempts;
+    P.use_ss = d.use_ss;
+    P.use_raman = d.use_raman && H_re && H_im;
+    P.use_plasma = d.use_plasma;
+    P.K = d.K;
+    P.avalanche = d.avalanche;
+    P.gamma = d.gamma;
+    P.f_R = d.f_R;
+    P.L = d.L;
+    P.tol = d.tol;
+    P.max_step = d.max_step;
+    P.min_step = d.min_step;
+    if (d.use_plasma) {
+        P.sigma_K = d.sigma_K;
+        P.inv_area = 1.0 / d.area;
+        P.Ui = d.Ui;
+        // the K1 launch casts these to float32 (media_physics.py:185-188)
+        P.rho_nt = (double)(float)d.rho_nt;
+        P.r_rec = (double)(float)(1.0 / d.tau_rec);
+        P.dt = (double)(float)d.dt;
+        P.half_sigma = d.sigma_drude / 2.0;
+        P.w0tc = d.omega0 * d.tau_c;
+        P.av_coef = d.avalanche ? d.sigma_drude / d.Ui : 0.0;
+        double gdt = -P.r_rec * P.dt;
+        gdt = gdt > 10.0 ? 10.0 : (gdt < -10.0 ? -10.0 : gdt);
+        P.e_rec = exp(gdt);
+        P.f_rec = fabs(gdt) < 1e-5 ? P.dt * (1.0 + 0.5 * gdt) : (P.e_rec - 1.0) / (-P.r_rec);
+    }
+    size_t o = 0;
+    double *dd = (double *)(hb + o);
+    for (int k = 0; k < nt; ++k) {
+        dd[k] = D_re[k];
+        dd[nt + k] = D_im[k];
+    }
+    P.D_re = (const double *)(db + o);
+    P.D_im = (const double *)(db + o) + nt;
+    o += align256(16 * (size_t)nt);
+    float *uf = (float *)(hb + o);
+    for (int k = 0; k < nt; ++k) uf[k] = (float)uppe_factor[k];
+    P.uppe = (const float *)(db + o);
+    o += align256(4 * (size_t)nt);
+    float2 *hv = (float2 *)(hb + o);
+    for (int k = 0; k < nt; ++k)
+        hv[k] = P.use_raman ? make_float2((float)H_re[k], (float)H_im[k])
+                            : make_float2(1.0f, 0.0f);
+    P.H = (const float2 *)(db + o);
+    o += align256(8 * (size_t)nt);
+    float2 *tv = (float2 *)(hb + o);
+    for (int k = 0; k < nt; ++k) {
+        const double ang = -2.0 * M_PI * (double)k / (double)nt;
+        tv[k] = make_float2((float)cos(ang), (float)sin(ang));
+    }
+    P.tw = (const float2 *)(db + o);
+    o += align256(8 * (size_t)nt);
+    P.A = (float2 *)(db + o);
+    o += align256(8 * (size_t)nt);
+    P.a5 = (float2 *)(db + o);
+    o += align256(8 * (size_t)nt);
+    P.v = (float2 *)(db + o);
+    o += align256(8 * (size_t)nt);
+    P.st = (Stat1D *)(db + o);
+    o += align256(sizeof(Stat1D));
+    return o;
 }
 
 int prepare(int n, const uppe1d_desc *descs, const double *D_re, const double *D_im,
@@ -1334,73 +1413,9 @@ int prepare(int n, const uppe1d_desc *descs, const double *D_re, const double *D
         const uppe1d_desc &d = descs[i];
         const int nt = d.nt;
         Inst1D &P = ins[i];
-        memset(&P, 0, sizeof P);
-        P.nt = nt;
-        P.max_hist = d.max_hist > 0 ? d.max_hist : 0;
-        P.max_attempts = d.max_attempts;
-        P.use_ss = d.use_ss;
-        P.use_raman = d.use_raman && H_re && H_im;
-        P.use_plasma = d.use_plasma;
-        P.K = d.K;
-        P.avalanche = d.avalanche;
-        P.gamma = d.gamma;
-        P.f_R = d.f_R;
-        P.L = d.L;
-        P.tol = d.tol;
-        P.max_step = d.max_step;
-        P.min_step = d.min_step;
-        if (d.use_plasma) {
-            P.sigma_K = d.sigma_K;
-            P.inv_area = 1.0 / d.area;
-            P.Ui = d.Ui;
-            // the K1 launch casts these to float32 (media_physics.py:185-188)
-            P.rho_nt = (double)(float)d.rho_nt;
-            P.r_rec = (double)(float)(1.0 / d.tau_rec);
-            P.dt = (double)(float)d.dt;
-            P.half_sigma = d.sigma_drude / 2.0;
-            P.w0tc = d.omega0 * d.tau_c;
-            P.av_coef = d.avalanche ? d.sigma_drude / d.Ui : 0.0;
-            double gdt = -P.r_rec * P.dt;
-            gdt = gdt > 10.0 ? 10.0 : (gdt < -10.0 ? -10.0 : gdt);
-            P.e_rec = exp(gdt);
-            P.f_rec = fabs(gdt) < 1e-5 ? P.dt * (1.0 + 0.5 * gdt) : (P.e_rec - 1.0) / (-P.r_rec);
-        }
-        char *hb = stage.data() + loc;
-        char *db = ws + off;
-        size_t o = 0;
-        double *dd = (double *)(hb + o);
-        for (int k = 0; k < nt; ++k) {
-            dd[k] = D_re[toff + k];
-            dd[nt + k] = D_im[toff + k];
-        }
-        P.D_re = (const double *)(db + o);
-        P.D_im = (const double *)(db + o) + nt;
-        o += align256(16 * (size_t)nt);
-        float *uf = (float *)(hb + o);
-        for (int k = 0; k < nt; ++k) uf[k] = (float)uppe_factor[toff + k];
-        P.uppe = (const float *)(db + o);
-        o += align256(4 * (size_t)nt);
-        float2 *hv = (float2 *)(hb + o);
-        for (int k = 0; k < nt; ++k)
-            hv[k] = P.use_raman ? make_float2((float)H_re[toff + k], (float)H_im[toff + k])
-                                : make_float2(1.0f, 0.0f);
-        P.H = (const float2 *)(db + o);
-        o += align256(8 * (size_t)nt);
-        float2 *tv = (float2 *)(hb + o);
-        for (int k = 0; k < nt; ++k) {
-            const double ang = -2.0 * M_PI * (double)k / (double)nt;
-            tv[k] = make_float2((float)cos(ang), (float)sin(ang));
-        }
-        P.tw = (const float2 *)(db + o);
-        o += align256(8 * (size_t)nt);
-        P.A = (float2 *)(db + o);
-        o += align256(8 * (size_t)nt);
-        P.a5 = (float2 *)(db + o);
-        o += align256(8 * (size_t)nt);
-        P.v = (float2 *)(db + o);
-        o += align256(8 * (size_t)nt);
-        P.st = (Stat1D *)(db + o);
-        o += align256(sizeof(Stat1D));
+        const size_t o = fill_instance(d, D_re + toff, D_im + toff, uppe_factor + toff,
+                                       H_re ? H_re + toff : nullptr, H_im ? H_im + toff : nullptr,
+                                       stage.data() + loc, ws + off, P);
         off += o;
         loc += o;
         P.a0 = (const float2 *)a0 + toff;
@@ -1459,6 +1474,21 @@ int launch(void *workspace, int n, int first, int count, int threads, int lds_ca
     }
 }
 
+void to_status(const Stat1D &x, uppe1d_status &o) {
+    o.z = x.z;
+    o.h = x.h;
+    o.err = x.err;
+    o.attempts = x.attempts;
+    o.accepted = x.accepted + x.forced;
+    o.rejected = x.rejected;
+    o.forced = x.forced;
+    o.nhist = x.nhist;
+    o.done = x.done;
+    o.nonfinite = x.nonfinite;
+    o.fsal_hits = x.fsal_hits;
+    o.overflow = x.hist_overflow | (x.rec_overflow << 1);
+}
+
 int status(void *workspace, int n, hipStream_t s, uppe1d_status *st) {
     UPPE_CHECK(workspace && st && n > 0, "uppe1d_batch_status: null argument");
     std::vector<Inst1D> ins(n);
@@ -1472,19 +1502,7 @@ int status(void *workspace, int n, hipStream_t s, uppe1d_status *st) {
     UPPE_HIP(hipStreamSynchronize(s));
     int bad = 0, rejected_cfg = 0;
     for (int i = 0; i < n; ++i) {
-        uppe1d_status &o = st[i];
-        o.z = hs[i].z;
-        o.h = hs[i].h;
-        o.err = hs[i].err;
-        o.attempts = hs[i].attempts;
-        o.accepted = hs[i].accepted + hs[i].forced;
-        o.rejected = hs[i].rejected;
-        o.forced = hs[i].forced;
-        o.nhist = hs[i].nhist;
-        o.done = hs[i].done;
-        o.nonfinite = hs[i].nonfinite;
-        o.fsal_hits = hs[i].fsal_hits;
-        o.overflow = hs[i].hist_overflow | (hs[i].rec_overflow << 1);
+        to_status(hs[i], st[i]);
         if (hs[i].nonfinite) bad = 1;
         if (hs[i].attempts < 0) rejected_cfg = 1;
     }
@@ -1648,6 +1666,95 @@ int adjoint(void *fwd_ws, int n, int index, int M, const double *hs, const void 
     return UPPE_OK;
 }
 
+// ---------------------------------------------------------------------------------------
+// Allocation-free single-pulse propagation (uppe1d_propagate_noalloc).  The workspace of one
+// instance (header, descriptor, tables, state, status) lives in a file-scope device array of
+// the largest supported length; the host staging of the tables in a file-scope host array.
+// The device address of the array is queried once per device with hipGetSymbolAddress (the
+// array's name is not a device address in host code) and kept per device.
+constexpr int NOALLOC_MAX_NT = 8192;
+constexpr size_t NOALLOC_BYTES = header_bytes(1) + inst_bytes(NOALLOC_MAX_NT);
+constexpr int NOALLOC_MAX_DEVICES = 64;
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+namespace {
+
+std::mutex g_noalloc_mu;                                  // serialises the host staging
+alignas(256) unsigned char g_noalloc_stage[inst_bytes(NOALLOC_MAX_NT)];
+void *g_noalloc_dev[NOALLOC_MAX_DEVICES];                  // device address of scratch
+
+int noalloc_scratch(void **ptr) {
+    int dev = 0;
+    UPPE_HIP(hipGetDevice(&dev));
+    UPPE_CHECK(dev >= 0 && dev < NOALLOC_MAX_DEVICES, "uppe1d_propagate_noalloc: device %d", dev);
+    if (!g_noalloc_dev[dev]) {
+        void *p = nullptr;
+        UPPE_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_noalloc_dev[dev] = p;
+    }
+    *ptr = g_noalloc_dev[dev];
+    return UPPE_OK;
+}
+
+int propagate_noalloc(const uppe1d_desc *d, const double *D_re, const double *D_im,
+                      const double *uppe_factor, const double *H_re, const double *H_im,
+                      const void *a0, void *hist, void *zhist, void *out_w,
+                      const uppe1d_record *rec, hipStream_t s, uppe1d_status *st) {
+    UPPE_CHECK(d && st && D_re && D_im && uppe_factor && a0, "uppe1d_propagate_noalloc: null argument");
+    UPPE_CHECK(d->nt <= NOALLOC_MAX_NT,
+               "uppe1d_propagate_noalloc: nt = %d exceeds the static workspace (at most %d)",
+               d->nt, NOALLOC_MAX_NT);
+    int r = check1d(*d);
+    if (r) return r;
+    UPPE_CHECK(d->max_hist <= 0 || (hist && zhist), "uppe1d: history buffers missing");
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    void *sp = nullptr;
+    r = noalloc_scratch(&sp);
+    if (r) return r;
+    char *ws = (char *)sp;
+    Inst1D P;
+    const size_t tables = fill_instance(*d, D_re, D_im, uppe_factor, H_re, H_im,
+                                        (char *)g_noalloc_stage, ws + header_bytes(1), P);
+    P.a0 = (const float2 *)a0;
+    P.hist = P.max_hist > 0 ? (float2 *)hist : nullptr;
+    P.zhist = P.max_hist > 0 ? (double *)zhist : nullptr;
+    P.outw = (float2 *)out_w;
+    if (rec) {
+        P.replay_hs = rec->replay_hs;
+        P.replay_M = rec->replay_hs ? rec->replay_M : 0;
+        P.hs = rec->hs;
+        P.hs_cap = rec->hs_cap;
+        P.ckpt = (float2 *)rec->ckpt;
+        P.ckpt_every = rec->ckpt_every;
+        P.ckpt_cap = rec->ckpt_cap;
+    }
+    Header1D h;
+    memset(&h, 0, sizeof h);
+    h.n = 1;
+    UPPE_HIP(hipMemcpyAsync(ws, &h, sizeof h, hipMemcpyHostToDevice, s));
+    UPPE_HIP(hipMemcpyAsync(ws + align256(sizeof(Header1D)), &P, sizeof P, hipMemcpyHostToDevice,
+                            s));
+    UPPE_HIP(hipMemcpyAsync(ws + header_bytes(1), g_noalloc_stage, tables,
+                            hipMemcpyHostToDevice, s));
+    // the instance counter is cleared by a kernel inside launch(); the status block is
+    // written whole by the kernel
+    r = launch(ws, 1, 0, 1, default_threads(d->nt), d->nt, 0, s);
+    if (r) return r;
+    Stat1D x;
+    UPPE_HIP(hipMemcpyAsync(&x, P.st, sizeof x, hipMemcpyDeviceToHost, s));
+    UPPE_HIP(hipStreamSynchronize(s));
+    UPPE_CHECK(x.attempts >= 0, "uppe1d: the instance does not fit the launch (nt vs threads)");
+    to_status(x, *st);
+    if (x.nonfinite) {
+        uppe::set_error("uppe1d: error norm became non-finite (NaN/Inf field)");
+        return UPPE_ENONFINITE;
+    }
+    return UPPE_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1709,6 +1816,14 @@ int uppe1d_propagate_slices(const uppe1d_desc *d, int nslices, const uppe1d_slic
                             void *stream, uppe1d_status *st) {
     return propagate_slices(d, nslices, slices, D_re, D_im, uppe_factor, H_re, H_im, a0,
                             workspace, hist, zhist, out_t, out_w, (hipStream_t)stream, st);
+}
+
+int uppe1d_propagate_noalloc(const uppe1d_desc *d, const double *D_re, const double *D_im,
+                             const double *uppe_factor, const double *H_re, const double *H_im,
+                             const void *a0, void *hist, void *zhist, void *out_w,
+                             const uppe1d_record *rec, void *stream, uppe1d_status *st) {
+    return propagate_noalloc(d, D_re, D_im, uppe_factor, H_re, H_im, a0, hist, zhist, out_w, rec,
+                             (hipStream_t)stream, st);
 }
 
 int uppe1d_propagate(const uppe1d_desc *d, const double *D_re, const double *D_im,

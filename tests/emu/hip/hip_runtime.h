@@ -103,6 +103,16 @@ inline hipError_t hipMemset(void *d, int v, size_t n) {
     return hipSuccess;
 }
 inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+inline hipError_t hipGetDevice(int *d) {
+    *d = 0;
+    return hipSuccess;
+}
+// a device symbol is an ordinary host object here
+#define HIP_SYMBOL(x) ((const void *)&(x))
+inline hipError_t hipGetSymbolAddress(void **p, const void *sym) {
+    *p = const_cast<void *>(sym);
+    return hipSuccess;
+}
 inline hipError_t hipDeviceSynchronize() { return hipSuccess; }
 
 // Stack switch: saves the callee-saved registers on the current stack, stores the stack
