@@ -714,6 +714,232 @@ __global__ __launch_bounds__(64 * LINES) void tpass_kernel(Fields3D f, const flo
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Persistent, software-pipelined t-pass for NT = 1024 without plasma (MODE 0: out = Y).  One
+// 512-thread workgroup per CU walks the 16-line groups; while it transforms group g, the rows
+// of group g + gridDim are already in flight into registers (32 per thread), so HBM reads
+// overlap the transforms.  8 wavefronts, two lines each, with 256 VGPRs available per wave.
+// Nothing in the loop waits on the vector-memory counter before the next group is staged:
+// the lane twiddles and the output factor (shock x mask) sit in LDS, the plasma bound reads
+// its LUT with scalar loads (the wave maximum is uniform), and deferred groups are listed in
+// LDS and appended to the global work list after the loop.
+// LDS: low area = 8 wave-private transform regions, which also hold lines 0..7 of the group
+// on entry (slot i at i * LS); high area (from HI) holds lines 8..15.  Each wave transforms
+// its half-1 line w while lines 8..15 wait in the high area, then takes line 8 + w from there
+// and leaves its half-1 result in that slot; half-2 results go to the low slots once every
+// transform is done.
+namespace pipe1024 {
+constexpr int T = 512;                        // threads
+constexpr int L = 16;                         // lines per group
+constexpr int PER = 1024 * L / T;             // rows per thread per group (32)
+constexpr int LS = 1028;                      // line slot pitch (float2)
+constexpr int HI = 8 * W1024::REGION + 2;     // high area base
+constexpr int TWO = HI + 8 * LS;              // lane twiddles A, B (2 x 1024 float2)
+constexpr int FO = TWO + 2048;                // output factor, 1024 floats
+constexpr int LDS = FO + 512;
+constexpr int MAXDEF = 64;                    // deferred groups per workgroup
+__device__ __forceinline__ int slot_in(int line) { return line < 8 ? line * LS : HI + (line - 8) * LS; }
+__device__ __forceinline__ int slot_out(int line) { return line < 8 ? HI + line * LS : (line - 8) * LS; }
+}  // namespace pipe1024
+
+// W1024 transforms with the lane twiddles in LDS (same arithmetic as w1024_inverse/forward)
+__device__ __forceinline__ void w1024_inverse_lds(float2 (&v)[16], float2 *buf, int lane,
+                                                  const float2 *twA, const float2 *twB) {
+    Dft<16, -1>::run(v);
+#pragma unroll
+    for (int k = 1; k < 16; ++k) v[k] = cmulc(v[k], twA[k * 64 + lane]);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) buf[k * W1024::P1 + lane] = v[k];
+    wave_lds_sync();
+    const int t1p = lane >> 2, la = lane & 3;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = buf[t1p * W1024::P1 + la + 4 * k];
+    wave_lds_sync();
+    Dft<16, -1>::run(v);
+#pragma unroll
+    for (int k = 1; k < 16; ++k) v[k] = cmulc(v[k], twB[k * 64 + lane]);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) buf[la * W1024::S1 + k * 16 + t1p] = v[k];
+    wave_lds_sync();
+    const int t1 = lane & 15, ug = lane >> 4;
+    constexpr float sc = 1.0f / 1024.0f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        float2 w[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) w[a] = buf[a * W1024::S1 + (4 * ug + j) * 16 + t1];
+        Dft<4, -1>::run(w);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) v[4 * j + b] = cscale(w[b], sc);
+    }
+    wave_lds_sync();
+}
+
+__device__ __forceinline__ void w1024_forward_lds(float2 (&v)[16], float2 *buf, int lane,
+                                                  const float2 *twA, const float2 *twB) {
+    const int t1 = lane & 15, ug = lane >> 4;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        float2 w[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) w[b] = v[4 * j + b];
+        Dft<4, 1>::run(w);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) buf[a * W1024::S1 + (4 * ug + j) * 16 + t1] = w[a];
+    }
+    wave_lds_sync();
+    const int t1p = lane >> 2, la = lane & 3;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = buf[la * W1024::S1 + k * 16 + t1p];
+    wave_lds_sync();
+#pragma unroll
+    for (int k = 1; k < 16; ++k) v[k] = cmul(v[k], twB[k * 64 + lane]);
+    Dft<16, 1>::run(v);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) buf[t1p * W1024::P1 + la + 4 * k] = v[k];
+    wave_lds_sync();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = buf[k * W1024::P1 + lane];
+    wave_lds_sync();
+#pragma unroll
+    for (int k = 1; k < 16; ++k) v[k] = cmul(v[k], twA[k * 64 + lane]);
+    Dft<16, 1>::run(v);
+}
+
+// plasma_matters<16> with the wave maximum made uniform, so that the LUT is read with scalar
+// loads; the same bound and decision.
+__device__ __forceinline__ bool plasma_matters_uniform(const float2 (&x)[16], const Phys3D &ph,
+                                                       const double *__restrict__ lut,
+                                                       double h) {
+    float imax = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) imax = fmaxf(imax, ph.a_scale2 * cabs2(x[k]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) imax = fmaxf(imax, __shfl_xor(imax, o, 64));
+    imax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(imax)));
+    if (!((double)imax > ph.lut_zero_below)) return false;
+    if (ph.av_coef != 0.0 || !ph.lut_bound_ok) return true;
+    double xl = log10((double)imax + 1e-30);
+    xl = fmin(fmax(xl, ph.lut_x0), ph.lut_x1);
+    int j = (int)((xl - ph.lut_x0) / ph.lut_dx);
+    j = j < 0 ? 0 : (j > ph.lut_n - 2 ? ph.lut_n - 2 : j);
+    j = __builtin_amdgcn_readfirstlane(j);
+    const double xj = ph.lut_x0 + j * ph.lut_dx;
+    const double xj1 = (j + 1 == ph.lut_n - 1) ? ph.lut_x1 : ph.lut_x0 + (j + 1) * ph.lut_dx;
+    const double f0 = lut[j], f1 = lut[j + 1];
+    const double wmax = (double)(float)((f1 - f0) / (xj1 - xj) * (xl - xj) + f0);
+    const double rho_max = ph.rho_nt * fmin(1.0, ph.f_rec * 1024.0 * wmax);
+    const double cbound = ph.Ui * ph.rho_nt * wmax / (2.0 * (double)imax) +
+                          ph.half_sigma * rho_max * (1.0 + ph.w0tc);
+    return !(h * cbound < 9.094947017729282e-13);
+}
+
+__global__ __launch_bounds__(pipe1024::T) void tpass_pipe1024_kernel(
+    Fields3D f, const float2 *in_fixed, int in_is_state, float2 *out, Tabs3D tb, Phys3D ph,
+    Ctl3D *ctl, int nx, int ny, int *list, int slot, int ngroups) {
+    using namespace pipe1024;
+    __shared__ float2 stage[LDS];
+    __shared__ int ndef, defl[MAXDEF];
+    if (ctl->done) return;
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    for (int i = tid; i < 1024; i += T) {
+        stage[TWO + i] = tb.twA[i];
+        stage[TWO + 1024 + i] = tb.twB[i];
+    }
+    float *fo = (float *)(stage + FO);
+    for (int i = tid; i < 1024; i += T)
+        fo[i] = ph.use_shock ? tb.shock_w[i] * tb.mask_w[i] : tb.mask_w[i];
+    if (tid == 0) ndef = 0;
+    const int cur = ctl->cur;
+    const float2 *in = in_is_state ? (cur ? f.B1 : f.B0) : in_fixed;
+    const int bpx = ny / L;
+    const int plane = nx * ny;
+    const int q = tid % L, row = tid / L;   // row 0..31: rows row + 32 c of the group
+    const int step32 = 32 * plane;
+    const double hh = (double)ctl->h;
+    float2 g[PER];
+    auto first_elem = [&](int grp) { return row * plane + (grp / bpx) * ny + (grp % bpx) * L + q; };
+    int grp = blockIdx.x;
+    if (grp < ngroups) {
+        int off = first_elem(grp);
+#pragma unroll
+        for (int c = 0; c < PER; ++c) {
+            g[c] = in[off];
+            off = opaque_i(off + step32);
+        }
+    }
+    const int sin_q = slot_in(q), sout_q = slot_out(q);
+    const float2 *twA = stage + TWO, *twB = stage + TWO + 1024;
+    float2 *buf = stage + wv * W1024::REGION;
+    for (; grp < ngroups; grp += gridDim.x) {
+        const int lo = first_elem(grp);
+        __syncthreads();   // the previous group's readers are done with the stage
+#pragma unroll
+        for (int c = 0; c < PER; ++c) stage[sin_q + row + 32 * c] = g[c];
+        __syncthreads();
+        float2 v[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) v[r] = stage[slot_in(wv) + lane + 64 * r];
+        __syncthreads();   // low area free: the transform regions alias it
+        const int gn = grp + gridDim.x;
+        if (gn < ngroups) {   // the next group's rows, in flight during this group's transforms
+            int off = first_elem(gn);
+#pragma unroll
+            for (int c = 0; c < PER; ++c) {
+                g[c] = in[off];
+                off = opaque_i(off + step32);
+            }
+        }
+        // half 1: line wv
+        w1024_inverse_lds(v, buf, lane, twA, twB);
+        bool pl = plasma_matters_uniform(v, ph, tb.lut, hh);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const float I = ph.a_scale2 * cabs2(v[k]);
+            const float kap = I * (ph.g_n2 + ph.g_n4 * I);
+            v[k] = make_float2(-kap * v[k].y, kap * v[k].x);
+        }
+        w1024_forward_lds(v, buf, lane, twA, twB);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) v[r] = cscale(v[r], fo[lane + 64 * r]);
+        // half 2: line 8 + wv, whose slot then takes the half-1 result (wave-private)
+        float2 u[16];
+        const int s2 = slot_in(8 + wv);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) u[r] = stage[s2 + lane + 64 * r];
+        wave_lds_sync();
+#pragma unroll
+        for (int r = 0; r < 16; ++r) stage[s2 + lane + 64 * r] = v[r];
+        w1024_inverse_lds(u, buf, lane, twA, twB);
+        pl = pl || plasma_matters_uniform(u, ph, tb.lut, hh);
+        if (__syncthreads_or(pl ? 1 : 0)) {
+            if (tid == 0) defl[ndef++] = grp;   // the plasma launch runs this group
+            continue;
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const float I = ph.a_scale2 * cabs2(u[k]);
+            const float kap = I * (ph.g_n2 + ph.g_n4 * I);
+            u[k] = make_float2(-kap * u[k].y, kap * u[k].x);
+        }
+        w1024_forward_lds(u, buf, lane, twA, twB);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) u[r] = cscale(u[r], fo[lane + 64 * r]);
+        __syncthreads();   // every transform region is done
+#pragma unroll
+        for (int r = 0; r < 16; ++r) stage[slot_out(8 + wv) + lane + 64 * r] = u[r];
+        __syncthreads();
+        int off = lo;
+#pragma unroll
+        for (int c = 0; c < PER; ++c) {
+            out[off] = stage[sout_q + row + 32 * c];
+            off = opaque_i(off + step32);
+        }
+    }
+    __syncthreads();
+    if (tid < ndef) list[atomicAdd(&ctl->plasma_groups[slot], 1)] = defl[tid];
+}
+
 // (x, y, t) input -> state B[cur] = FFT_t(A0) / a_scale in the plane-major layout.
 template <int NT, int LINES>
 __global__ __launch_bounds__(64 * LINES) void load_kernel(const float2 *__restrict__ a0,
@@ -2056,6 +2282,18 @@ int launch_tpass(uppe3d_plan *p, int mode, int slot, const float2 *in, int in_is
     // while uppe3d_run's checks see an empty list a small grid saves launch time
     const int pg = p->plasma_small ? 8 : p->ncu;
     const dim3 grid(groups), pgrid(groups < pg ? groups : pg);
+    if (p->d.nt == 1024 && mode == 0) {
+        // persistent: one workgroup per CU, at most MAXDEF groups each
+        int pgr = p->ncu;
+        if (pgr * pipe1024::MAXDEF < groups) pgr = (groups + pipe1024::MAXDEF - 1) / pipe1024::MAXDEF;
+        if (pgr > groups) pgr = groups;
+        tpass_pipe1024_kernel<<<pgr, pipe1024::T, 0, s>>>(p->f, in, in_is_state, out, p->tb, p->ph,
+                                                         p->ctl, nx, ny, p->list, slot, groups);
+        tpass_kernel<1024, 16, 0, true><<<pgrid, 1024, 0, s>>>(p->f, in, in_is_state, out, p->tb,
+                                                              p->ph, p->ctl, nx, ny, p->list, slot);
+        UPPE_HIP(hipGetLastError());
+        return UPPE_OK;
+    }
     switch (p->d.nt) {
         LAUNCH_T(128, 16)
         LAUNCH_T(256, 16)

@@ -36,27 +36,27 @@ def make(nx, nt, energy, n4=True, shock=False, guard=0.0, tau=40e-15, w0=1e-3, h
 
 
 CASES = [
-    # (nx, nt, energy, n4, shock, L, tol, tolerance)
-    ("16x128 kerr+hoke", 16, 128, 0.8e-3, True, False, 0.02, 1e-5, 2e-5),
-    ("32x256 kerr", 32, 256, 0.8e-3, False, False, 0.5, 1e-5, 2e-5),
-    ("32x256 shock+hoke", 32, 256, 0.8e-3, True, True, 0.01, 1e-5, 2e-5),
+    # (nx, nt, energy, n4, shock, L, tol, tolerance): every case at 1e-5 (measured 5e-7 to 7e-6)
+    ("16x128 kerr+hoke", 16, 128, 0.8e-3, True, False, 0.02, 1e-5, 1e-5),
+    ("32x256 kerr", 32, 256, 0.8e-3, False, False, 0.5, 1e-5, 1e-5),
+    ("32x256 shock+hoke", 32, 256, 0.8e-3, True, True, 0.01, 1e-5, 1e-5),
     # narrow window (half width 1 mm, w0 = 0.3 mm): peak 6e17 W/m^2 exercises the FP64 affine
     # scan; peak 5e18 W/m^2 makes W dt > exp(gdt) and takes the serial recurrence
-    ("32x256 plasma scan", 32, 256, 4e-3, False, False, 0.005, 1e-5, 2e-5),
-    ("32x256 plasma stress", 32, 256, 30e-3, False, True, 0.001, 1e-4, 1e-3),
+    ("32x256 plasma scan", 32, 256, 4e-3, False, False, 0.005, 1e-5, 1e-5),
+    ("32x256 plasma stress", 32, 256, 30e-3, False, True, 0.001, 1e-4, 1e-5),
     # the longest supported lines (NT = 2048, 32 values per lane)
-    ("16x2048 kerr+hoke", 16, 2048, 0.8e-3, True, False, 0.02, 1e-5, 2e-5),
+    ("16x2048 kerr+hoke", 16, 2048, 0.8e-3, True, False, 0.02, 1e-5, 1e-5),
     # NT = 1024 (two-team t-pass): one 16-line tile per workgroup; groups deferred to the
     # 8-line plasma instantiation (scan and serial recurrence, with shock)
-    ("16x1024 kerr+hoke", 16, 1024, 0.8e-3, True, False, 0.02, 1e-5, 2e-5),
-    ("32x1024 plasma scan+shock", 32, 1024, 4e-3, False, True, 0.005, 1e-5, 2e-5),
-    ("32x1024 plasma stress", 32, 1024, 30e-3, False, True, 0.001, 1e-4, 1e-3),
+    ("16x1024 kerr+hoke", 16, 1024, 0.8e-3, True, False, 0.02, 1e-5, 1e-5),
+    ("32x1024 plasma scan+shock", 32, 1024, 4e-3, False, True, 0.005, 1e-5, 1e-5),
+    ("32x1024 plasma stress", 32, 1024, 30e-3, False, True, 0.001, 1e-4, 1e-5),
     # planes beyond one workgroup's LDS (column pass + row kernel)
-    ("256x128 kerr+hoke+shock", 256, 128, 0.8e-3, True, True, 0.02, 1e-5, 2e-5),
+    ("256x128 kerr+hoke+shock", 256, 128, 0.8e-3, True, True, 0.02, 1e-5, 1e-5),
     # fused silica, peak 1e17 W/m^2: multiphoton and avalanche ionisation (the general FP64
     # density recurrence, avalanche rate sigma/Ui I), NT = 256 and NT = 1024
-    ("fs 32x256 avalanche", 32, 256, 0.6e-3, False, False, 2e-4, 1e-5, 2e-5),
-    ("fs 32x1024 avalanche+shock", 32, 1024, 0.6e-3, False, True, 2e-4, 1e-5, 2e-5),
+    ("fs 32x256 avalanche", 32, 256, 0.6e-3, False, False, 2e-4, 1e-5, 1e-5),
+    ("fs 32x1024 avalanche+shock", 32, 1024, 0.6e-3, False, True, 2e-4, 1e-5, 1e-5),
 ]
 
 
@@ -91,7 +91,7 @@ def test_free_running(gpu):
     print("attempts gpu", st.attempts, "oracle", len(log), "rel L2", rel_l2(Aw_g, Aw_o))
     assert abs(z_g - 0.3) < 1e-12
     assert abs(st.attempts - len(log)) <= 2
-    assert rel_l2(Aw_g, Aw_o) < 2e-5
+    assert rel_l2(Aw_g, Aw_o) < 1e-5
 
 
 def gpu_attempt_log(solver, A0, L, tol, max_step=1e-2, min_step=1e-9):
@@ -190,4 +190,4 @@ def test_plasma_work_on_the_small_grid(gpu):
     _, _, Aw_g = solver._finish(False)
     e = rel_l2(Aw_g, Aw_o)
     print("rel L2", e)
-    assert e < 2e-5
+    assert e < 1e-5
