@@ -70,6 +70,9 @@ struct Ctl3D {
     int l2;                 // error norm: 0 max-norm ratio, 1 L2-norm ratio
     int plasma_seen;        // workgroups deferred to plasma since uppe3d_begin
     double sumA, sumD;      // L2 norm: sum |A4|^2 and sum |FFT2(Y4 - Y5)|^2 of the attempt
+    float plasma_thr;       // lines whose peak intensity (state units) reaches this get the
+                            // plasma term at the current h (set with every new h)
+    int pad2_;
 };
 
 struct Tabs3D {
@@ -465,21 +468,56 @@ __device__ __forceinline__ void plasma_walk(float2 (&x)[R], int lane, const Phys
 // When h times that bound is below 2^-40 the plasma contribution to every stage field is
 // far below one FP32 ulp and the line is treated as plasma-free.  A line whose intensities
 // all sit below the LUT's zero region has W = 0 exactly.
+// The bound grows with I_max (the host checks W and W / I on the LUT grid; between grid points
+// the interpolated W / I may fall by at most the factor 10^(grid step), far below 2), so the
+// decision is a threshold on I_max: plasma_threshold finds, once per step size, the smallest
+// float I_max whose bound reaches 2^-41 (a factor 2 below the cutoff covers the
+// interpolation), and a line takes the plasma term iff its peak intensity reaches it.
+__device__ bool plasma_bound_hit(float imax, const Phys3D &ph, const double *__restrict__ lut,
+                                 double h, int nt) {
+    if (!((double)imax > ph.lut_zero_below)) return false;
+    if (ph.av_coef != 0.0 || !ph.lut_bound_ok) return true;
+    const double wmax = (double)lut_rate(imax, lut, ph);
+    const double rho_max = ph.rho_nt * fmin(1.0, ph.f_rec * (double)nt * wmax);
+    const double cbound = ph.Ui * ph.rho_nt * wmax / (2.0 * (double)imax) +
+                          ph.half_sigma * rho_max * (1.0 + ph.w0tc);
+    return !(h * cbound < 4.547473508864641e-13);
+}
+
+// One wavefront: the smallest non-negative float I (searched on its bit pattern, which orders
+// non-negative floats) with plasma_bound_hit(I); +inf if there is none.  Each round tests 64
+// equally spaced candidates, so about six rounds cover the 2^31 patterns.
+__device__ void plasma_threshold(Ctl3D *c, const Phys3D &ph, const double *__restrict__ lut,
+                                 int nt, int lane) {
+    const double h = c->h;
+    unsigned lo = 0u, hi = 0x7f800000u;   // every pattern below lo misses; hi hits (or +inf)
+    while (lo < hi) {
+        const unsigned long long range = hi - lo;
+        const unsigned b = lo + (unsigned)((range * (unsigned long long)lane) / 64ull);
+        const bool hit = plasma_bound_hit(__uint_as_float(b), ph, lut, h, nt);
+        const unsigned long long m = __ballot(hit);
+        if (m) {
+            const int k = __ffsll((long long)m) - 1;
+            const unsigned bk = __shfl(b, k, 64);
+            const unsigned bp = k > 0 ? __shfl(b, k - 1, 64) + 1u : lo;
+            hi = bk;
+            lo = k > 0 ? bp : bk;
+        } else {
+            lo = __shfl(b, 63, 64) + 1u;
+        }
+    }
+    if (lane == 0) c->plasma_thr = __uint_as_float(hi);
+}
+
+// peak intensity (state units x a_scale^2) of the line held by the wavefront
 template <int R>
-__device__ __forceinline__ bool plasma_matters(const float2 (&x)[R], const Phys3D &ph,
-                                               const double *__restrict__ lut, double h) {
+__device__ __forceinline__ float line_imax(const float2 (&x)[R], const Phys3D &ph) {
     float imax = 0.0f;
 #pragma unroll
     for (int k = 0; k < R; ++k) imax = fmaxf(imax, ph.a_scale2 * cabs2(x[k]));
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) imax = fmaxf(imax, __shfl_xor(imax, o, 64));
-    if (!((double)imax > ph.lut_zero_below)) return false;
-    if (ph.av_coef != 0.0 || !ph.lut_bound_ok) return true;
-    const double wmax = (double)lut_rate_inl(imax, lut, ph);
-    const double rho_max = ph.rho_nt * fmin(1.0, ph.f_rec * (64.0 * R) * wmax);
-    const double cbound = ph.Ui * ph.rho_nt * wmax / (2.0 * (double)imax) +
-                          ph.half_sigma * rho_max * (1.0 + ph.w0tc);
-    return !(h * cbound < 9.094947017729282e-13);
+    return imax;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -591,7 +629,7 @@ __device__ __forceinline__ void tpass_group(int blk, float2 *stage, Fields3D f,
         wave_ifft_t<R>(v, lane, rl, tb.tw_t);
     }
 
-    bool plasma = plasma_matters<R>(v, ph, tb.lut, (double)ctl->h);
+    bool plasma = line_imax<R>(v, ph) >= ctl->plasma_thr;
     if (!PLASMA) {
         const int defer = __syncthreads_or(plasma ? 1 : 0);
         if (defer) {
@@ -720,9 +758,9 @@ __global__ __launch_bounds__(64 * LINES) void tpass_kernel(Fields3D f, const flo
 // of group g + gridDim are already in flight into registers (32 per thread), so HBM reads
 // overlap the transforms.  8 wavefronts, two lines each, with 256 VGPRs available per wave.
 // Nothing in the loop waits on the vector-memory counter before the next group is staged:
-// the lane twiddles and the output factor (shock x mask) sit in LDS, the plasma bound reads
-// its LUT with scalar loads (the wave maximum is uniform), and deferred groups are listed in
-// LDS and appended to the global work list after the loop.
+// the lane twiddles and the output factor (shock x mask) sit in LDS, the plasma decision is a
+// comparison with the step's threshold intensity, and deferred groups are listed in LDS and
+// appended to the global work list after the loop.
 // LDS: low area = 8 wave-private transform regions, which also hold lines 0..7 of the group
 // on entry (slot i at i * LS); high area (from HI) holds lines 8..15.  Each wave transforms
 // its half-1 line w while lines 8..15 wait in the high area, then takes line 8 + w from there
@@ -806,34 +844,6 @@ __device__ __forceinline__ void w1024_forward_lds(float2 (&v)[16], float2 *buf, 
     Dft<16, 1>::run(v);
 }
 
-// plasma_matters<16> with the wave maximum made uniform, so that the LUT is read with scalar
-// loads; the same bound and decision.
-__device__ __forceinline__ bool plasma_matters_uniform(const float2 (&x)[16], const Phys3D &ph,
-                                                       const double *__restrict__ lut,
-                                                       double h) {
-    float imax = 0.0f;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) imax = fmaxf(imax, ph.a_scale2 * cabs2(x[k]));
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) imax = fmaxf(imax, __shfl_xor(imax, o, 64));
-    imax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(imax)));
-    if (!((double)imax > ph.lut_zero_below)) return false;
-    if (ph.av_coef != 0.0 || !ph.lut_bound_ok) return true;
-    double xl = log10((double)imax + 1e-30);
-    xl = fmin(fmax(xl, ph.lut_x0), ph.lut_x1);
-    int j = (int)((xl - ph.lut_x0) / ph.lut_dx);
-    j = j < 0 ? 0 : (j > ph.lut_n - 2 ? ph.lut_n - 2 : j);
-    j = __builtin_amdgcn_readfirstlane(j);
-    const double xj = ph.lut_x0 + j * ph.lut_dx;
-    const double xj1 = (j + 1 == ph.lut_n - 1) ? ph.lut_x1 : ph.lut_x0 + (j + 1) * ph.lut_dx;
-    const double f0 = lut[j], f1 = lut[j + 1];
-    const double wmax = (double)(float)((f1 - f0) / (xj1 - xj) * (xl - xj) + f0);
-    const double rho_max = ph.rho_nt * fmin(1.0, ph.f_rec * 1024.0 * wmax);
-    const double cbound = ph.Ui * ph.rho_nt * wmax / (2.0 * (double)imax) +
-                          ph.half_sigma * rho_max * (1.0 + ph.w0tc);
-    return !(h * cbound < 9.094947017729282e-13);
-}
-
 __global__ __launch_bounds__(pipe1024::T) void tpass_pipe1024_kernel(
     Fields3D f, const float2 *in_fixed, int in_is_state, float2 *out, Tabs3D tb, Phys3D ph,
     Ctl3D *ctl, int nx, int ny, int *list, int slot, int ngroups) {
@@ -856,7 +866,7 @@ __global__ __launch_bounds__(pipe1024::T) void tpass_pipe1024_kernel(
     const int plane = nx * ny;
     const int q = tid % L, row = tid / L;   // row 0..31: rows row + 32 c of the group
     const int step32 = 32 * plane;
-    const double hh = (double)ctl->h;
+    const float thr = ctl->plasma_thr;
     float2 g[PER];
     auto first_elem = [&](int grp) { return row * plane + (grp / bpx) * ny + (grp % bpx) * L + q; };
     int grp = blockIdx.x;
@@ -892,7 +902,7 @@ __global__ __launch_bounds__(pipe1024::T) void tpass_pipe1024_kernel(
         }
         // half 1: line wv
         w1024_inverse_lds(v, buf, lane, twA, twB);
-        bool pl = plasma_matters_uniform(v, ph, tb.lut, hh);
+        bool pl = line_imax<16>(v, ph) >= thr;
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
             const float I = ph.a_scale2 * cabs2(v[k]);
@@ -911,7 +921,7 @@ __global__ __launch_bounds__(pipe1024::T) void tpass_pipe1024_kernel(
 #pragma unroll
         for (int r = 0; r < 16; ++r) stage[s2 + lane + 64 * r] = v[r];
         w1024_inverse_lds(u, buf, lane, twA, twB);
-        pl = pl || plasma_matters_uniform(u, ph, tb.lut, hh);
+        pl = pl || line_imax<16>(u, ph) >= thr;
         if (__syncthreads_or(pl ? 1 : 0)) {
             if (tid == 0) defl[ndef++] = grp;   // the plasma launch runs this group
             continue;
@@ -1160,7 +1170,8 @@ __device__ __forceinline__ float2 eprop_k(float kp2, int w, float s, const Tabs3
 
 // Per-plane factors exp(i s d_w), exp(-i s k_ref) for the current step h (s = h / 2), phases
 // in FP64.  Launched whenever h is set (begin, controller, replay).
-__global__ void phase_kernel(const Ctl3D *ctl, Tabs3D tb, int nt) {
+__global__ void phase_kernel(Ctl3D *ctl, Tabs3D tb, Phys3D ph, int nt) {
+    if (blockIdx.x == 0 && threadIdx.x < 64) plasma_threshold(ctl, ph, tb.lut, nt, threadIdx.x);
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= nt) return;
     const double s = 0.5 * ctl->h;
@@ -1587,8 +1598,8 @@ __device__ __forceinline__ void line_store(const float2 (&v)[NT / 64], float2 *_
 // Xb (+)= N'(X)^H Yb for the Kerr (+ HOKE) term of the t-pass: with x = iFFT_t(X),
 // g = iFFT_t(shock mask Yb), I = s2 |x|^2, phi = g2 I + g4 I^2, phi' = g2 + 2 g4 I and
 // gk = -i g (adjoint of the factor i):  Xb = FFT_t((phi + phi' I) gk + phi' s2 x^2 conj(gk)).
-// A line whose plasma term matters at FP32 resolution (plasma_matters) sets ctl->adj_plasma:
-// the plasma adjoint is not provided for 3D.
+// A line whose plasma term matters at FP32 resolution (peak intensity at or above the step's
+// threshold, plasma_threshold) sets ctl->adj_plasma: the plasma adjoint is not provided for 3D.
 template <int NT, int LINES>
 __global__ __launch_bounds__(64 * LINES) void tadj_kernel(const float2 *X, const float2 *Yb,
                                                           float2 *out, const float2 *acc,
@@ -1613,7 +1624,7 @@ __global__ __launch_bounds__(64 * LINES) void tadj_kernel(const float2 *X, const
     }
     wave_ifft_t<R>(x, lane, rl, tb.tw_t);
     wave_ifft_t<R>(g, lane, rl, tb.tw_t);
-    if (plasma_matters<R>(x, ph, tb.lut, h) && lane == 0) atomicOr(&ctl->adj_plasma, 1);
+    if (line_imax<R>(x, ph) >= ctl->plasma_thr && lane == 0) atomicOr(&ctl->adj_plasma, 1);
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const float2 a = x[k];
@@ -2123,10 +2134,12 @@ __device__ void controller(Ctl3D *c) {
 
 // End of an attempt: the controller (thread 0), then the per-plane FP64 phases of the new h
 // (what phase_kernel computes) by the whole workgroup, in one launch.
-__global__ __launch_bounds__(1024) void ctl_phase_kernel(Ctl3D *c, Tabs3D tb, int nt) {
+__global__ __launch_bounds__(1024) void ctl_phase_kernel(Ctl3D *c, Tabs3D tb, Phys3D ph,
+                                                        int nt) {
     if (threadIdx.x == 0) controller(c);
     __syncthreads();
     if (c->done) return;
+    if (threadIdx.x < 64) plasma_threshold(c, ph, tb.lut, nt, threadIdx.x);
     const double s = 0.5 * c->h;
     for (int w = threadIdx.x; w < nt; w += blockDim.x) {
         tb.ph_d[w] = cis_d(s * tb.d64[w]);
@@ -2434,7 +2447,7 @@ int enqueue_attempt(uppe3d_plan *p, hipStream_t s, hipEvent_t *ev = nullptr) {
     RET(mark());
     RET(launch_xy(p, 6, s));                          // P6
     RET(mark());
-    ctl_phase_kernel<<<1, 1024, 0, s>>>(p->ctl, p->tb, p->d.nt);
+    ctl_phase_kernel<<<1, 1024, 0, s>>>(p->ctl, p->tb, p->ph, p->d.nt);
     UPPE_HIP(hipGetLastError());
     RET(mark());
     return UPPE_OK;
@@ -2688,7 +2701,8 @@ int uppe3d_begin(uppe3d_plan *p, double L, double tol, double max_step, double m
     UPPE_CHECK(tol > 0.0 && max_step > 0.0 && min_step >= 0.0,
                "uppe3d_begin: tol and max_step must be positive");
     begin_kernel<<<1, 1, 0, (hipStream_t)stream>>>(p->ctl, L, tol, max_step, min_step);
-    phase_kernel<<<(p->d.nt + 255) / 256, 256, 0, (hipStream_t)stream>>>(p->ctl, p->tb, p->d.nt);
+    phase_kernel<<<(p->d.nt + 255) / 256, 256, 0, (hipStream_t)stream>>>(p->ctl, p->tb, p->ph,
+                                                                       p->d.nt);
     UPPE_HIP(hipGetLastError());
     p->plasma_small = 0;   // full plasma grid until a run window has seen an empty work list
     return UPPE_OK;
@@ -2735,7 +2749,7 @@ int uppe3d_replay_attempt(uppe3d_plan *p, double h, int accept, void *stream) {
     UPPE_CHECK(p != nullptr && h > 0.0, "uppe3d_replay_attempt: bad argument");
     hipStream_t s = (hipStream_t)stream;
     replay_set_kernel<<<1, 1, 0, s>>>(p->ctl, h, accept, h);
-    phase_kernel<<<(p->d.nt + 255) / 256, 256, 0, s>>>(p->ctl, p->tb, p->d.nt);
+    phase_kernel<<<(p->d.nt + 255) / 256, 256, 0, s>>>(p->ctl, p->tb, p->ph, p->d.nt);
     UPPE_HIP(hipGetLastError());
     return enqueue_attempt(p, s);
 }
@@ -3010,7 +3024,7 @@ int uppe3d_adjoint_step(uppe3d_plan *p, double h, const void *B, const void *X2,
     const float hf = (float)h;
     // propagator tables of this step size
     adj_h_kernel<<<1, 1, 0, s>>>(p->ctl, h);
-    phase_kernel<<<(p->d.nt + 255) / 256, 256, 0, s>>>(p->ctl, p->tb, p->d.nt);
+    phase_kernel<<<(p->d.nt + 255) / 256, 256, 0, s>>>(p->ctl, p->tb, p->ph, p->d.nt);
     UPPE_HIP(hipGetLastError());
     RET(xadj(p, 5, a, hf, s));
     RET(tadj(p, (const float2 *)X4, a.Yb, a.Xb, nullptr, h, s));
