@@ -525,7 +525,9 @@ __device__ __forceinline__ float line_imax(const float2 (&x)[R], const Phys3D &p
 // LINES lines (fixed x, y0..y0+LINES-1), one wavefront per line; HBM is read and written in
 // rows of LINES*8 contiguous bytes and transposed through a small LDS stage, CH chunks of 64
 // omega rows at a time.
-// MODE 0: out = Y.  MODE 1 (final stage): out = Y4 - Y, and the next state
+// MODE 0: out = Y.  MODE 2 (final stage): out = Y, and the next state
+// B' = X * filter(omega) * exp(-damp(x,y) h) is written to the spare state buffer on the way;
+// MODE 1 (final stage, with the difference): as MODE 2 with out = Y4 - Y.  The next state
 // B' = X * filter(omega) * exp(-damp(x,y) h) is written to the spare state buffer on the way.
 //
 // Two instantiations per (NT, MODE): PLASMA = false has no plasma code (so the FFTs keep their
@@ -589,7 +591,7 @@ __device__ __forceinline__ void tpass_group(int blk, float2 *stage, Fields3D f,
         // LDS transposition of the current one
         float absorb = 1.0f;
         float2 *bnew = nullptr;
-        if (MODE == 1) {
+        if (MODE != 0) {
             absorb = expf(-tb.damp[col0 + q] * (float)ctl->h);
             bnew = cur ? f.B0 : f.B1;
         }
@@ -603,7 +605,7 @@ __device__ __forceinline__ void tpass_group(int blk, float2 *stage, Fields3D f,
                 for (int cc = 0; cc < CH; ++cc)
                     gn[cc] = in[lo + 64 * (c0 + CH + cc) * plane];
             }
-            if (MODE == 1) {
+            if (MODE != 0) {
 #pragma unroll
                 for (int cc = 0; cc < CH; ++cc) {
                     const int w = row + 64 * (c0 + cc);
@@ -774,7 +776,8 @@ constexpr int LS = 1028;                      // line slot pitch (float2)
 constexpr int HI = 8 * W1024::REGION + 2;     // high area base
 constexpr int TWO = HI + 8 * LS;              // lane twiddles A, B (2 x 1024 float2)
 constexpr int FO = TWO + 2048;                // output factor, 1024 floats
-constexpr int LDS = FO + 512;
+constexpr int FI = FO + 512;                  // anti-alias filter (final stage), 1024 floats
+constexpr int LDS = FI + 512;
 constexpr int MAXDEF = 64;                    // deferred groups per workgroup
 __device__ __forceinline__ int slot_in(int line) { return line < 8 ? line * LS : HI + (line - 8) * LS; }
 __device__ __forceinline__ int slot_out(int line) { return line < 8 ? HI + line * LS : (line - 8) * LS; }
@@ -844,6 +847,10 @@ __device__ __forceinline__ void w1024_forward_lds(float2 (&v)[16], float2 *buf, 
     Dft<16, 1>::run(v);
 }
 
+// MODE 0: out = Y.  MODE 2 (final stage): out = Y, and the next state
+// B' = X * filter(omega) * exp(-damp(x,y) h) is written from the group's rows on the way (the
+// error estimate then takes FFT2(Y4 - Y) in the xy pass).
+template <int MODE>
 __global__ __launch_bounds__(pipe1024::T) void tpass_pipe1024_kernel(
     Fields3D f, const float2 *in_fixed, int in_is_state, float2 *out, Tabs3D tb, Phys3D ph,
     Ctl3D *ctl, int nx, int ny, int *list, int slot, int ngroups) {
@@ -859,6 +866,9 @@ __global__ __launch_bounds__(pipe1024::T) void tpass_pipe1024_kernel(
     float *fo = (float *)(stage + FO);
     for (int i = tid; i < 1024; i += T)
         fo[i] = ph.use_shock ? tb.shock_w[i] * tb.mask_w[i] : tb.mask_w[i];
+    float *fi = (float *)(stage + FI);
+    if (MODE == 2)
+        for (int i = tid; i < 1024; i += T) fi[i] = tb.filt_w[i];
     if (tid == 0) ndef = 0;
     const int cur = ctl->cur;
     const float2 *in = in_is_state ? (cur ? f.B1 : f.B0) : in_fixed;
@@ -884,6 +894,17 @@ __global__ __launch_bounds__(pipe1024::T) void tpass_pipe1024_kernel(
     for (; grp < ngroups; grp += gridDim.x) {
         const int lo = first_elem(grp);
         __syncthreads();   // the previous group's readers are done with the stage
+        if (MODE == 2) {
+            const float absorb = expf(-tb.damp[(grp / bpx) * ny + (grp % bpx) * L + q] *
+                                      (float)ctl->h);
+            float2 *bnew = cur ? f.B0 : f.B1;
+            int off = lo;
+#pragma unroll
+            for (int c = 0; c < PER; ++c) {
+                bnew[off] = cscale(g[c], fi[row + 32 * c] * absorb);
+                off = opaque_i(off + step32);
+            }
+        }
 #pragma unroll
         for (int c = 0; c < PER; ++c) stage[sin_q + row + 32 * c] = g[c];
         __syncthreads();
@@ -1442,6 +1463,18 @@ __device__ __forceinline__ void fft2_128(float2 (&v)[16], const float2 *__restri
     fwd128(v, lds + wv * x128::WREG, lane, tw);
 }
 
+// FFT2 of the difference srcA - srcB of two planes (row-major, HBM)
+__device__ __forceinline__ void fft2_128_diff(float2 (&v)[16], const float2 *__restrict__ srcA,
+                                              const float2 *__restrict__ srcB, float2 *lds,
+                                              int wv, int lane, const float2 *tw) {
+    const int row = 8 * wv + (lane >> 3), j = lane & 7;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = csub(srcA[row * 128 + j + 8 * r], srcB[row * 128 + j + 8 * r]);
+    fwd128(v, lds + wv * x128::WREG, lane, tw);
+    rows_to_cols(v, lds, wv, lane);
+    fwd128(v, lds + wv * x128::WREG, lane, tw);
+}
+
 // k-space layout -> iFFT2 / (128 * 128) -> plane (row-major, HBM)
 __device__ __forceinline__ void ifft2_128(float2 (&v)[16], float2 *__restrict__ dst,
                                           float2 *lds, int wv, int lane, const float2 *tw) {
@@ -1454,7 +1487,8 @@ __device__ __forceinline__ void ifft2_128(float2 (&v)[16], float2 *__restrict__ 
     for (int r = 0; r < 16; ++r) dst[row * 128 + j + 8 * r] = cscale(v[r], s);
 }
 
-// The xy passes of xypass_kernel (MODE 2..6, same arithmetic) for 128 x 128 planes.
+// The xy passes of xypass_kernel (MODE 2..6, same arithmetic) for 128 x 128 planes, and MODE 7:
+// MODE 6 on FFT2(Y4 - Yw), for a final t-pass that leaves Y5 in Yw instead of the difference.
 template <int MODE>
 __global__ __launch_bounds__(x128::T) void xy128_kernel(Fields3D f, Tabs3D tb, Ctl3D *ctl) {
     __shared__ float2 lds[x128::LDS];
@@ -1470,8 +1504,9 @@ __global__ __launch_bounds__(x128::T) void xy128_kernel(Fields3D f, Tabs3D tb, C
     const float2 *const Yp = (MODE == 5 ? f.Y4 : f.Yw) + pb;
     const int ky = 8 * wv + (lane >> 3), kxb = 2 * (lane & 7);
     float2 b[16];
-    fft2_128(b, Yp, lds, wv, lane, tb.tw128);
-    if (MODE == 6) {
+    if (MODE == 7) fft2_128_diff(b, f.Y4 + pb, f.Yw + pb, lds, wv, lane, tb.tw128);
+    else fft2_128(b, Yp, lds, wv, lane, tb.tw128);
+    if (MODE == 6 || MODE == 7) {
         unsigned int m = 0;
         float sq = 0.0f;
 #pragma unroll
@@ -2295,15 +2330,22 @@ int launch_tpass(uppe3d_plan *p, int mode, int slot, const float2 *in, int in_is
     // while uppe3d_run's checks see an empty list a small grid saves launch time
     const int pg = p->plasma_small ? 8 : p->ncu;
     const dim3 grid(groups), pgrid(groups < pg ? groups : pg);
-    if (p->d.nt == 1024 && mode == 0) {
+    if (p->d.nt == 1024 && (mode == 0 || mode == 2)) {
         // persistent: one workgroup per CU, at most MAXDEF groups each
         int pgr = p->ncu;
         if (pgr * pipe1024::MAXDEF < groups) pgr = (groups + pipe1024::MAXDEF - 1) / pipe1024::MAXDEF;
         if (pgr > groups) pgr = groups;
-        tpass_pipe1024_kernel<<<pgr, pipe1024::T, 0, s>>>(p->f, in, in_is_state, out, p->tb, p->ph,
-                                                         p->ctl, nx, ny, p->list, slot, groups);
-        tpass_kernel<1024, 16, 0, true><<<pgrid, 1024, 0, s>>>(p->f, in, in_is_state, out, p->tb,
-                                                              p->ph, p->ctl, nx, ny, p->list, slot);
+        if (mode == 0) {
+            tpass_pipe1024_kernel<0><<<pgr, pipe1024::T, 0, s>>>(
+                p->f, in, in_is_state, out, p->tb, p->ph, p->ctl, nx, ny, p->list, slot, groups);
+            tpass_kernel<1024, 16, 0, true><<<pgrid, 1024, 0, s>>>(
+                p->f, in, in_is_state, out, p->tb, p->ph, p->ctl, nx, ny, p->list, slot);
+        } else {
+            tpass_pipe1024_kernel<2><<<pgr, pipe1024::T, 0, s>>>(
+                p->f, in, in_is_state, out, p->tb, p->ph, p->ctl, nx, ny, p->list, slot, groups);
+            tpass_kernel<1024, 16, 2, true><<<pgrid, 1024, 0, s>>>(
+                p->f, in, in_is_state, out, p->tb, p->ph, p->ctl, nx, ny, p->list, slot);
+        }
         UPPE_HIP(hipGetLastError());
         return UPPE_OK;
     }
@@ -2391,6 +2433,7 @@ int launch_xy(uppe3d_plan *p, int mode, hipStream_t s) {
                 case 3: xy128_kernel<3><<<grid, block, 0, s>>>(p->f, p->tb, p->ctl); break;
                 case 4: xy128_kernel<4><<<grid, block, 0, s>>>(p->f, p->tb, p->ctl); break;
                 case 5: xy128_kernel<5><<<grid, block, 0, s>>>(p->f, p->tb, p->ctl); break;
+                case 7: xy128_kernel<7><<<grid, block, 0, s>>>(p->f, p->tb, p->ctl); break;
                 default: xy128_kernel<6><<<grid, block, 0, s>>>(p->f, p->tb, p->ctl); break;
             }
             UPPE_HIP(hipGetLastError());
@@ -2443,9 +2486,11 @@ int enqueue_attempt(uppe3d_plan *p, hipStream_t s, hipEvent_t *ev = nullptr) {
     RET(mark());
     RET(launch_xy(p, 5, s));                          // P5
     RET(mark());
-    RET(launch_tpass(p, 1, 4, f.X, 0, f.Yw, s));        // T5 (D and next state)
+    // T5 (next state, and D = Y4 - Y5 or, on 128 x 128 x 1024, Y5 with D formed in P6)
+    const bool split = p->d.nt == 1024 && p->d.nx == 128;
+    RET(launch_tpass(p, split ? 2 : 1, 4, f.X, 0, f.Yw, s));
     RET(mark());
-    RET(launch_xy(p, 6, s));                          // P6
+    RET(launch_xy(p, split ? 7 : 6, s));              // P6
     RET(mark());
     ctl_phase_kernel<<<1, 1024, 0, s>>>(p->ctl, p->tb, p->ph, p->d.nt);
     UPPE_HIP(hipGetLastError());

@@ -154,8 +154,12 @@ def test_c1_grid_replay(gpu):
     announce("3d replay 128x128x1024 attempts=%d" % len(log))
     z_g, f_g, Aw_g = solver.propagate_replay(A0, log)
     e = rel_l2(Aw_g, Aw_o)
-    print("C1 rel L2", e)
+    errs_o = np.array([l[1] for l in log])
+    errs_g = np.array(solver.replay_errors)
+    print("C1 rel L2", e, "error norms gpu/oracle", (errs_g / errs_o)[:6])
     assert e < 1e-5
+    # the embedded error estimate of every attempt (the controller's input)
+    assert np.all(np.abs(errs_g - errs_o) <= 1e-2 * errs_o)
 
 
 def test_plasma_work_on_the_small_grid(gpu):
