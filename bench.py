@@ -29,20 +29,35 @@ HBM_PEAK_GBS = 8000.0
 SLOTS = ["T1", "P2", "T2", "P3", "T3", "P4", "T4", "P5", "T5", "P6", "CTL"]
 FIELD_PASSES = {"T1": 2, "P2": 5, "T2": 2, "P3": 5, "T3": 2, "P4": 5, "T4": 2, "P5": 3,
                 "T5": 4, "P6": 1, "CTL": 0}
+# on 128 x 128 x 1024 the final t-pass leaves Y5 (1 F read, 2 F written) and the error pass
+# reads Y4 and Y5
+FIELD_PASSES_SPLIT = dict(FIELD_PASSES, T5=3, P6=2)
 GROUPS = {"tpass": ["T1", "T2", "T3", "T4"], "tpass_final": ["T5"], "xy_stage": ["P2", "P4"],
           "xy_stage_noexp": ["P3"], "xy_final": ["P5"], "xy_err": ["P6"], "ctl": ["CTL"]}
 
 
-def measured_traffic(nx, nt):
-    """HBM bytes per t-pass launch (FETCH_SIZE + WRITE_SIZE, calibrated against a copy kernel)
-    from the committed rocprofv3 counter summary of this grid, or None."""
+def measured_traffic(nx, nt, kernel="tpass"):
+    """HBM bytes per launch of one kernel group (FETCH_SIZE + WRITE_SIZE, calibrated against a
+    copy kernel) from the committed rocprofv3 counter summary of this grid, or None."""
     path = os.path.join(ROOT, "profiles", "uppe3d_%dx%dx%d_hbm_traffic.md" % (nx, nx, nt))
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("tpass_bytes_per_launch ="):
+        if line.startswith("%s_bytes_per_launch =" % kernel):
             return int(line.split("=")[1])
     return None
+
+
+def cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo), or the platform's processor string."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
 
 
 def cpu_baseline(nx, nt, attempts, workers):
@@ -57,6 +72,18 @@ def cpu_baseline(nx, nt, attempts, workers):
     z, f, Aw, log = orc.propagate(A0, 0.5, tol=1e-5, max_attempts=attempts)
     dt = time.perf_counter() - t0
     return len(log) / dt, len(log), dt
+
+
+def oracle_argon_1d(pressure_bar):
+    """The FP64 oracle of the 1D argon model (project_amd.workloads.argon_solver_1d: K = 8
+    plasma with sigma_8 = 1e-130 per atm, rho_nt and tau_c scaled with pressure)."""
+    from oracle import uppe1d as o1
+    p_atm = pressure_bar * 1e5 / 101325.0
+    pl = o1.Plasma1D(K=8, sigma_K=1e-130 * p_atm, rho_nt=2.45e25 * p_atm, tau_c=190e-15 / p_atm)
+    return o1.Oracle1D(o1.Params1D(material="argon_gas", n2=1e-23 * pressure_bar,
+                                   beam_radius=20e-6, center_wavelength=800e-9, use_raman=False,
+                                   use_self_steepening=True, gas_P=pressure_bar * 1e5,
+                                   gas_T=293.15, plasma=pl))
 
 
 def bench_config0(cpu=True):
@@ -79,13 +106,7 @@ def bench_config0(cpu=True):
            "gpu_s_per_solve": round(gpu, 5), "attempts": st.attempts,
            "accepted": st.accepted, "z_steps_per_s": round(st.attempts / gpu, 1)}
     if cpu:
-        from oracle import uppe1d as o1
-        orc = o1.Oracle1D(o1.Params1D(material="argon_gas", n2=1e-23, beam_radius=20e-6,
-                                      center_wavelength=800e-9, use_raman=False,
-                                      use_self_steepening=True, gas_P=1e5, gas_T=293.15,
-                                      plasma=o1.Plasma1D(K=8, sigma_K=1e-130 * 1e5 / 101325.0,
-                                                         rho_nt=2.45e25 * 1e5 / 101325.0,
-                                                         tau_c=190e-15 * 101325.0 / 1e5)))
+        orc = oracle_argon_1d(1.0)
         t0 = time.perf_counter()
         zo, Aho, wo, log = orc.propagate(a0, t, 1.0, tol=1e-6)
         cpu_s = time.perf_counter() - t0
@@ -95,9 +116,27 @@ def bench_config0(cpu=True):
     return out
 
 
-def bench_batched(n=256, L=1.0):
+def batched_counters():
+    """FP32-VALU and LDS figures of the batched sweep kernel from the committed rocprofv3
+    counter summary, or None."""
+    path = os.path.join(ROOT, "profiles", "uppe1d_batch256_counters.md")
+    if not os.path.exists(path):
+        return None
+    out = {}
+    for line in open(path):
+        if line.startswith("bench:"):
+            k, v = line[6:].split("=")
+            out[k.strip()] = float(v)
+    return out or None
+
+
+def bench_batched(n=256, L=1.0, cpu=True):
     """Config 2: 256 instances, N_t ~ U{1024..8192}, E ~ logU[0.1, 10] uJ, GDD ~ U[+-500] fs^2,
-    P ~ U[0.5, 3] bar, L 1 m, tol 1e-6, one persistent launch (autotuned), graph replay."""
+    P ~ U[0.5, 3] bar, L 1 m, tol 1e-6, one persistent launch (autotuned), graph replay.
+    One workgroup per instance, all resident at once, so the sweep cannot end before its
+    longest instance: that instance (largest attempts x N_t) is also timed alone.  The CPU
+    baseline is the FP64 oracle's time per attempt at each N_t (one full solve per N_t on one
+    host core) applied to the attempts of every instance."""
     import torch
     from project_amd import workloads as wl
     from project_amd.batched import BatchedSweep1D
@@ -118,15 +157,42 @@ def bench_batched(n=256, L=1.0):
     done = sum(x.done for x in st)
     # streaming model: 18 complex64 field transfers of N_t points per attempt
     alg_bytes = sum(x.attempts * 18 * 8 * p["nt"] for x, p in zip(st, params))
-    return {"workload": "batched 1D argon sweep, %d instances, Nt in {1024..8192}, L %g m, "
-                        "tol 1e-6" % (n, L),
-            "solves_per_s": round(n / secs, 2), "seconds": round(secs, 4),
-            "z_steps_per_s": round(att / secs, 1), "attempts_total": att,
-            "instances_done": done, "plan": [list(g_) for g_ in best.groups],
-            "autotune": [[list(map(list, p)), round(t_, 4)] for (p, t_) in sw.tuning],
-            "hbm_model_GBps": round(alg_bytes / secs / 1e9, 1),
-            "hbm_model_frac": round(alg_bytes / secs / 1e9 / HBM_PEAK_GBS, 4),
-            "nt_counts": {str(k): int((nts == k).sum()) for k in sorted(set(nts))}}
+    atts = np.array([x.attempts for x in st])
+    crit = int(np.argmax(atts * nts))
+    one = BatchedSweep1D([solvers[crit]], [inputs[crit]], L, tol=1e-6)
+    one.run()
+    st1, crit_s = one.run()
+    out = {"workload": "batched 1D argon sweep, %d instances, Nt in {1024..8192}, L %g m, "
+                       "tol 1e-6" % (n, L),
+           "solves_per_s": round(n / secs, 2), "seconds": round(secs, 4),
+           "z_steps_per_s": round(att / secs, 1), "attempts_total": att,
+           "instances_done": done, "plan": [list(g_) for g_ in best.groups],
+           "autotune": [[list(map(list, p)), round(t_, 4)] for (p, t_) in sw.tuning],
+           "critical_path": {"instance": crit, "nt": int(nts[crit]),
+                             "attempts": int(st1[0].attempts), "alone_s": round(crit_s, 4),
+                             "sweep_over_alone": round(secs / crit_s, 3)},
+           "hbm_model_GBps": round(alg_bytes / secs / 1e9, 1),
+           "hbm_model_frac": round(alg_bytes / secs / 1e9 / HBM_PEAK_GBS, 4),
+           "counters": batched_counters(),
+           "nt_counts": {str(k): int((nts == k).sum()) for k in sorted(set(nts))}}
+    if cpu:
+        per_att, sample = {}, []
+        for nt in sorted(set(nts)):
+            k = int(np.nonzero(nts == nt)[0][0])
+            t0 = time.perf_counter()
+            o = oracle_argon_1d(params[k]["pressure_bar"])
+            _, _, _, log = o.propagate(inputs[k][1], inputs[k][0], L, tol=1e-6)
+            dt = time.perf_counter() - t0
+            per_att[nt] = dt / len(log)
+            sample.append("instance %d (Nt %d, %d attempts, %.2f s)" % (k, nt, len(log), dt))
+        proj = float(sum(a * per_att[nt] for a, nt in zip(atts, nts)))
+        out["cpu_baseline"] = {"value": round(n / proj, 4), "unit": "solves/s", "cores": 1,
+                               "kind": "port", "projected_sweep_s": round(proj, 1),
+                               "sample": "FP64 oracle, one full solve per Nt: " + "; ".join(sample)
+                                         + "; per-attempt time x the GPU's attempts of every "
+                                           "instance"}
+        out["speedup_vs_cpu"] = round(proj / secs, 1)
+    return out
 
 
 def bench_cavity(passes=2, slices=20, cpu=True):
@@ -252,9 +318,11 @@ def bench_realfield(nx=128, nt=1024, attempts=10, cpu=True):
     return out
 
 
-def bench_adjoint():
-    """Config 3: gradient of ||S_out - S_target||^2 for the config-0 setup (N_t = 4096, 1 m):
-    one recorded forward solve and one backward pass, timed separately."""
+def bench_adjoint(sigma8=1e-121, L=0.2):
+    """1D adjoint: gradient of ||S_out - S_target||^2 with respect to the input spectral phase
+    for the config-0 pulse (N_t = 4096) with K = 8 plasma of cross section sigma8 per atm over
+    L: one recorded forward solve and one backward pass, timed separately.  The defaults are the
+    plasma case of the 1D adjoint GPU test."""
     import torch
     from project_amd import workloads as wl
     from project_amd.adjoint import (FrozenPropagator1D, UPPEPropagate1D, spectral_loss,
@@ -264,7 +332,7 @@ def bench_adjoint():
     base = np.fft.fft(a0)
     dev = torch.device("cuda")
     S_t = torch.from_numpy(super_gaussian_target(omega, base)).to(dev)
-    prop = FrozenPropagator1D(wl.argon_solver_1d(1.0), t, 1.0, tol=1e-6)
+    prop = FrozenPropagator1D(wl.argon_solver_1d(1.0, sigma8_per_atm=sigma8), t, L, tol=1e-6)
     D0, U0, g0 = prop.tables()
     A0w = torch.from_numpy(base).to(dev).to(torch.complex64)
     prop.run(A0w, D0, U0, g0)          # adaptive run, freezes the step sequence
@@ -287,8 +355,9 @@ def bench_adjoint():
         fw.append(t1 - t0)
         bw.append(t2 - t1)
     f, b = min(fw), min(bw)
-    return {"workload": "adjoint 1D config-0 setup: dL/d(phase[4096], energy, pressure), "
-                        "%d frozen steps, checkpoints every 16" % prop.M,
+    return {"workload": "adjoint 1D, config-0 pulse (Nt 4096), K=8 plasma sigma8 %g, L %g m: "
+                        "dL/d(phase[4096]), %d frozen steps, checkpoints every 16"
+                        % (sigma8, L, prop.M),
             "forward_s": round(f, 5), "backward_s": round(b, 5),
             "adjoint_over_forward": round(b / f, 3)}
 
@@ -337,6 +406,77 @@ def bench_adjoint3d(nx=256, nt=2048, L=0.3):
             "memory_GB": round(torch.cuda.max_memory_allocated() / 1e9, 2)}
 
 
+def time_attempts(solver, A0, steps, warmup):
+    """`steps` attempts of a loaded 3D solver from a fixed start (load, begin, `warmup`
+    attempts through the production driver): first as the production driver launches them (no
+    events inside the attempts, HIP events at the two ends), then again from the same start with
+    an event between consecutive launch groups (per-kernel times, ms summed over the attempts)."""
+    import torch
+    from project_amd import _lib
+    lib = _lib.load()
+    s = _lib.stream_handle()
+    stw = _lib.Uppe3DStatus()
+
+    def start():
+        solver.load(A0)
+        _lib.check(lib.uppe3d_begin(solver._plan, 0.5, 1e-5, 1e-2, 1e-9, s), "uppe3d_begin")
+        # warm-up through propagate's driver (uppe3d_run, one status check at its end), which
+        # also sizes the plasma launch grid from what it saw
+        _lib.check(lib.uppe3d_run(solver._plan, warmup, warmup, s, ctypes.byref(stw)),
+                   "uppe3d_run")
+
+    start()
+    torch.cuda.synchronize()
+    st0 = solver.status()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    ev0.record()
+    for _ in range(steps):
+        _lib.check(lib.uppe3d_attempt(solver._plan, s), "uppe3d_attempt")
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    st1 = solver.status()
+    if st1.done:
+        raise RuntimeError("propagation finished inside the timed region; lower --steps")
+    start()
+    kms = (ctypes.c_float * 11)()
+    tot = ctypes.c_float(0.0)
+    _lib.check(lib.uppe3d_attempt_timed(solver._plan, steps, s, kms, ctypes.byref(tot)),
+               "uppe3d_attempt_timed")
+    torch.cuda.synchronize()
+    return {"st0": st0, "st1": st1, "ms_plain": ev0.elapsed_time(ev1), "kms": list(kms),
+            "ms_event": float(tot.value), "wall": wall}
+
+
+def bench_plasma3d(steps=50, warmup=5, nx=128, nt=1024):
+    """The headline attempt with the plasma term active on the full grid: argon 1 bar (Kerr,
+    HOKE, LUT ionisation, FP64 plasma recurrence), 4 mJ 40 fs on a 0.3 mm waist in a 2 mm
+    window (peak ~6e17 W/m^2, the narrow-waist scan case of the 3D parity tests), tol 1e-5:
+    z-steps/s, the t-pass line groups that took the plasma instantiation, per-kernel times."""
+    from project_amd import media_physics as mp, workloads as wl
+    from project_amd.solver_3d import ERK43IP_UPPE_3D_Optimized
+    x, y, t = wl.grid3d(nx, nt, half_width=1e-3)
+    A0 = wl.beam3d(x, y, t, energy=4e-3, w0=0.3e-3)
+    med = mp.Argon(800e-9, pressure_atm=1e5 / 101325, tunnel_guard=0.0)
+    solver = ERK43IP_UPPE_3D_Optimized(x, y, t, med)
+    r = time_attempts(solver, A0, steps, warmup)
+    st0, st1 = r["st0"], r["st1"]
+    att = st1.attempts - st0.attempts
+    groups = 5 * att * nx * nx // (16 if nt <= 1024 else 8)
+    kms = r["kms"]
+    per = {g: round(sum(kms[SLOTS.index(k)] for k in sl) / (len(sl) * steps), 5)
+           for g, sl in GROUPS.items()}
+    return {"workload": "3+1D argon 1 bar, Kerr+HOKE+LUT plasma (tunnel_guard 0), 4 mJ 40 fs "
+                        "800 nm, w0 0.3 mm, window 2 mm, %dx%dx%d, tol 1e-5" % (nx, nx, nt),
+            "steps": steps, "z_steps_per_s": round(1000.0 * steps / r["ms_plain"], 2),
+            "ms_per_step": round(r["ms_plain"] / steps, 4),
+            "z_start": st0.z, "z_end": st1.z, "attempts": att,
+            "plasma_groups_frac": round((st1.plasma_groups - st0.plasma_groups) / groups, 4),
+            "kernel_avg_ms": per}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -348,8 +488,8 @@ def main():
                     help="oracle attempts timed for the CPU baseline (0 skips it)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the 1D, batched and adjoint legs")
-    ap.add_argument("--only", choices=["mpc_3d", "realfield_3d", "cavity_1d", "batched_1d",
-                                       "config0_1d", "adjoint_3d"],
+    ap.add_argument("--only", choices=["plasma_3d", "mpc_3d", "realfield_3d", "cavity_1d",
+                                       "batched_1d", "config0_1d", "adjoint_1d", "adjoint_3d"],
                     help="run one side leg alone and print its JSON (not the headline line)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="before the run, copy a MIB-MiB float32 buffer 3 times (a plain copy "
@@ -361,8 +501,10 @@ def main():
         sys.exit(2)
     if args.only:   # one side leg alone (for profiling it); prints that leg's JSON
         legs = {"mpc_3d": bench_mpc, "realfield_3d": lambda: bench_realfield(cpu=False),
-                "cavity_1d": lambda: bench_cavity(cpu=False), "batched_1d": bench_batched,
-                "config0_1d": lambda: bench_config0(cpu=False), "adjoint_3d": bench_adjoint3d}
+                "cavity_1d": lambda: bench_cavity(cpu=False),
+                "batched_1d": lambda: bench_batched(cpu=False),
+                "config0_1d": lambda: bench_config0(cpu=False), "adjoint_1d": bench_adjoint,
+                "adjoint_3d": bench_adjoint3d, "plasma_3d": bench_plasma3d}
         print(json.dumps({args.only: legs[args.only]()}), flush=True)
         return
 
@@ -383,53 +525,19 @@ def main():
     A0 = wl.beam3d(x, y, t)
     med = mp.Argon(800e-9, pressure_atm=1e5 / 101325, tunnel_guard=0.0)
     solver = ERK43IP_UPPE_3D_Optimized(x, y, t, med)
-    solver.load(A0)
-    s = _lib.stream_handle()
-    _lib.check(lib.uppe3d_begin(solver._plan, 0.5, 1e-5, 1e-2, 1e-9, s), "uppe3d_begin")
-    # warm-up through the production driver (propagate's uppe3d_run, one status check at its
-    # end), which also sizes the plasma re-run grid from what it saw
-    stw = _lib.Uppe3DStatus()
-    _lib.check(lib.uppe3d_run(solver._plan, args.warmup, args.warmup, s, ctypes.byref(stw)),
-               "uppe3d_run")
-    torch.cuda.synchronize()
-    st0 = solver.status()
-
-    # the metric: K attempts launched as the production driver launches them (no events inside
-    # the attempts), bracketed by device synchronisations, HIP events at the two ends
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    w0 = time.perf_counter()
-    ev0.record()
-    for _ in range(args.steps):
-        _lib.check(lib.uppe3d_attempt(solver._plan, s), "uppe3d_attempt")
-    ev1.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - w0
-    st1 = solver.status()
-    ms_plain = ev0.elapsed_time(ev1)
-    if st1.done:
-        raise RuntimeError("propagation finished inside the timed region; lower --steps")
-    # per-kernel times: the same attempts again from the same start (reload, begin, warm-up),
-    # with an event between consecutive launch groups
-    solver.load(A0)
-    _lib.check(lib.uppe3d_begin(solver._plan, 0.5, 1e-5, 1e-2, 1e-9, s), "uppe3d_begin")
-    _lib.check(lib.uppe3d_run(solver._plan, args.warmup, args.warmup, s, ctypes.byref(stw)),
-               "uppe3d_run")
-    kms = (ctypes.c_float * 11)()
-    tot = ctypes.c_float(0.0)
-    _lib.check(lib.uppe3d_attempt_timed(solver._plan, args.steps, s, kms, ctypes.byref(tot)),
-               "uppe3d_attempt_timed")
-    torch.cuda.synchronize()
-    ms_event = float(tot.value)
+    r = time_attempts(solver, A0, args.steps, args.warmup)
+    st0, st1, ms_plain, kms, ms_event, wall = (r[k] for k in ("st0", "st1", "ms_plain", "kms",
+                                                              "ms_event", "wall"))
     ms_per_step = ms_plain / args.steps
     steps_per_s = 1000.0 / ms_per_step
 
     field_bytes = 8.0 * nx * nx * nt
+    passes = FIELD_PASSES_SPLIT if (nx, nt) == (128, 1024) else FIELD_PASSES
     kernels = {}
     for g, slots in GROUPS.items():
         ms = sum(kms[SLOTS.index(k)] for k in slots)
         launches = len(slots) * args.steps
-        byts = FIELD_PASSES[slots[0]] * field_bytes
+        byts = passes[slots[0]] * field_bytes
         avg_ms = ms / launches
         kernels[g] = {"avg_ms": avg_ms, "share": ms / ms_event,
                       "bytes_per_launch": byts,
@@ -437,7 +545,7 @@ def main():
     dom = max((g for g in kernels if kernels[g]["bytes_per_launch"] > 0),
               key=lambda g: kernels[g]["share"])
     d = kernels[dom]
-    traffic = measured_traffic(nx, nt) if dom == "tpass" else None
+    traffic = measured_traffic(nx, nt, dom)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(d["GBps"], 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(d["GBps"] / HBM_PEAK_GBS, 4),
                 "traffic": traffic}
@@ -451,10 +559,14 @@ def main():
     if args.cpu_attempts > 0:
         workers = min(os.cpu_count() or 1, 16)
         v, n, secs = cpu_baseline(nx, nt, args.cpu_attempts, workers)
+        v1, n1, secs1 = cpu_baseline(nx, nt, 1, 1)
         cpu = {"value": round(v, 5), "unit": "z-steps/s", "cores": workers, "kind": "port",
                "sample": "%d ERK4(3)-IP attempt(s) of the same workload from z=0 with the FP64 "
                          "oracle (NumPy/SciPy pocketfft, scipy workers=%d), %.1f s"
-                         % (n, workers, secs)}
+                         % (n, workers, secs),
+               "cpu_model": cpu_model(),
+               "one_core": {"value": round(v1, 5), "unit": "z-steps/s", "cores": 1,
+                            "sample": "%d attempt, scipy workers=1, %.1f s" % (n1, secs1)}}
 
     out = {
         "metric": "3+1D UPPE z-steps/s (128^2x1024) + HBM GB/s",
@@ -470,7 +582,8 @@ def main():
         "dtype": "fp32 (complex64; FP64 plasma recurrence and error norm)",
         "data": "synthetic (seeded Gaussian beam + 1e-4 spatial noise)",
         "config": {"workload": "3+1D envelope UPPE ERK4(3)-IP, argon 1 bar (Kerr+HOKE+LUT "
-                               "plasma), 0.8 mJ 40 fs 800 nm, w0 1 mm, tol 1e-5, L 0.5 m",
+                               "plasma, tunnel exponent without the 1e-20 guard: "
+                               "tunnel_guard=0), 0.8 mJ 40 fs 800 nm, w0 1 mm, tol 1e-5, L 0.5 m",
                    "grid": [nx, nx, nt], "z_start": st0.z, "z_end": st1.z,
                    "attempts": st1.attempts - st0.attempts,
                    "accepted": st1.accepted - st0.accepted,
@@ -486,18 +599,20 @@ def main():
         out["speedup_vs_cpu"] = round(steps_per_s / cpu["value"], 1)
     if not args.no_extra:
         out["config0_1d"] = bench_config0(cpu=args.cpu_attempts > 0)
-        out["batched_1d"] = bench_batched()
+        out["batched_1d"] = bench_batched(cpu=args.cpu_attempts > 0)
         out["cavity_1d"] = bench_cavity(cpu=args.cpu_attempts > 0)
-        for key, leg in (("mpc_3d", bench_mpc),
+        for key, leg in (("plasma_3d", bench_plasma3d), ("mpc_3d", bench_mpc),
                          ("realfield_3d", lambda: bench_realfield(cpu=args.cpu_attempts > 0))):
             try:
                 out[key] = leg()
             except Exception as e:   # a failing side leg is reported, not fatal to the line
                 out[key] = {"error": "%s: %s" % (type(e).__name__, e)}
-        # the 1D adjoint leg (config-0 setup, checkpoints every 16) is the computation of the
-        # GPU test case that is kept off the GPU (tests/test_gpu_adjoint.py, config3)
-        out["adjoint_1d"] = {"skipped": "config-3 adjoint case kept off the GPU after a fault "
-                                        "with an experimental, since removed, 1D kernel build"}
+        # the 1D adjoint is timed on the plasma case of its GPU test; the config-3 case (1 m,
+        # sigma8 1e-130) is kept off the GPU (tests/test_gpu_adjoint.py)
+        out["adjoint_1d"] = bench_adjoint()
+        out["adjoint_1d"]["config3"] = ("not timed: the 1 m, sigma8 1e-130 case is kept off "
+                                        "the GPU after a fault with an experimental, since "
+                                        "removed, 1D kernel build")
         try:
             out["adjoint_3d"] = bench_adjoint3d()
         except NotImplementedError as e:

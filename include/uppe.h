@@ -59,6 +59,7 @@ typedef struct {
 typedef struct {
     double z, h, err;      /* position, next step size, last attempt's error norm */
     int attempts, accepted, rejected, done, nonfinite, fail_count, cur;
+    int plasma_groups;     /* t-pass line groups evaluated with the plasma term since begin */
 } uppe3d_status;
 
 /* Bytes of device workspace a plan needs (tables + 7 fields + control block). */

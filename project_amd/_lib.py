@@ -27,7 +27,8 @@ class Uppe3DDesc(ctypes.Structure):
 class Uppe3DStatus(ctypes.Structure):
     _fields_ = [("z", c_double), ("h", c_double), ("err", c_double), ("attempts", c_int),
                 ("accepted", c_int), ("rejected", c_int), ("done", c_int),
-                ("nonfinite", c_int), ("fail_count", c_int), ("cur", c_int)]
+                ("nonfinite", c_int), ("fail_count", c_int), ("cur", c_int),
+                ("plasma_groups", c_int)]
 
 
 class UppeRf3DDesc(ctypes.Structure):

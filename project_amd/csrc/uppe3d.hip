@@ -2819,6 +2819,7 @@ int uppe3d_status_get(uppe3d_plan *p, void *stream, uppe3d_status *st) {
     st->nonfinite = c.nonfinite;
     st->fail_count = c.fail_count;
     st->cur = c.cur;
+    st->plasma_groups = c.plasma_seen;
     return UPPE_OK;
 }
 
