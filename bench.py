@@ -25,7 +25,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# launch slots of one attempt and their algorithmic complex64-field transfers
+# launch slots of one attempt (uppe3d_attempt_timed) and their algorithmic complex64-field
+# transfers.  The 11-launch attempt (stage combinations in (kx, ky, omega), 31 field passes):
 SLOTS = ["T1", "P2", "T2", "P3", "T3", "P4", "T4", "P5", "T5", "P6", "CTL"]
 FIELD_PASSES = {"T1": 2, "P2": 5, "T2": 2, "P3": 5, "T3": 2, "P4": 5, "T4": 2, "P5": 3,
                 "T5": 4, "P6": 1, "CTL": 0}
@@ -34,6 +35,21 @@ FIELD_PASSES = {"T1": 2, "P2": 5, "T2": 2, "P3": 5, "T3": 2, "P4": 5, "T4": 2, "
 FIELD_PASSES_SPLIT = dict(FIELD_PASSES, T5=3, P6=2)
 GROUPS = {"tpass": ["T1", "T2", "T3", "T4"], "tpass_final": ["T5"], "xy_stage": ["P2", "P4"],
           "xy_stage_noexp": ["P3"], "xy_final": ["P5"], "xy_err": ["P6"], "ctl": ["CTL"]}
+# the restructured attempt (stage fields in (x, y, omega), fused t-passes, 21 field passes):
+# T1 reads B, writes Y1; XY-A reads B, Y1, writes X2, Zh; T23 reads X2, Zh, writes two
+# increments; XY-B reads B and the increments, writes X4, R'; T45 reads X4, R', writes B', D;
+# XY-E reads B', D
+SLOTS_R = ["T1", "XYA", "T23", "XYB", "T45", "XYE", "CTL", "-", "-", "-", "-"]
+FIELD_PASSES_R = {"T1": 2, "XYA": 4, "T23": 4, "XYB": 5, "T45": 4, "XYE": 2, "CTL": 0}
+GROUPS_R = {"tpass": ["T1"], "tpass_fused": ["T23", "T45"], "xy_a": ["XYA"], "xy_b": ["XYB"],
+            "xy_err": ["XYE"], "ctl": ["CTL"]}
+
+
+def attempt_tables(path, nx, nt):
+    """(slots, field passes per slot, kernel groups) of the attempt dataflow `path`."""
+    if path == "restructured":
+        return SLOTS_R, FIELD_PASSES_R, GROUPS_R
+    return SLOTS, (FIELD_PASSES_SPLIT if (nx, nt) == (128, 1024) else FIELD_PASSES), GROUPS
 
 
 def measured_traffic(nx, nt, kernel="tpass"):
@@ -447,7 +463,7 @@ def time_attempts(solver, A0, steps, warmup):
                "uppe3d_attempt_timed")
     torch.cuda.synchronize()
     return {"st0": st0, "st1": st1, "ms_plain": ev0.elapsed_time(ev1), "kms": list(kms),
-            "ms_event": float(tot.value), "wall": wall}
+            "ms_event": float(tot.value), "wall": wall, "path": solver.attempt_path()}
 
 
 def bench_plasma3d(steps=50, warmup=5, nx=128, nt=1024):
@@ -464,17 +480,23 @@ def bench_plasma3d(steps=50, warmup=5, nx=128, nt=1024):
     r = time_attempts(solver, A0, steps, warmup)
     st0, st1 = r["st0"], r["st1"]
     att = st1.attempts - st0.attempts
-    groups = 5 * att * nx * nx // (16 if nt <= 1024 else 8)
+    slots, _, grp = attempt_tables(r["path"], nx, nt)
+    # line groups a t-pass launch deals out (plasma deferral unit): 5 t-pass launches per
+    # attempt of 16-line groups, or, restructured, T1 (16 lines) and two fused passes (8)
+    if r["path"] == "restructured":
+        groups = att * (nx * nx // 16 + 2 * nx * nx // (8 if nt >= 1024 else 16))
+    else:
+        groups = 5 * att * nx * nx // (16 if nt <= 1024 else 8)
     kms = r["kms"]
-    per = {g: round(sum(kms[SLOTS.index(k)] for k in sl) / (len(sl) * steps), 5)
-           for g, sl in GROUPS.items()}
+    per = {g: round(sum(kms[slots.index(k)] for k in sl) / (len(sl) * steps), 5)
+           for g, sl in grp.items()}
     return {"workload": "3+1D argon 1 bar, Kerr+HOKE+LUT plasma (tunnel_guard 0), 4 mJ 40 fs "
                         "800 nm, w0 0.3 mm, window 2 mm, %dx%dx%d, tol 1e-5" % (nx, nx, nt),
             "steps": steps, "z_steps_per_s": round(1000.0 * steps / r["ms_plain"], 2),
             "ms_per_step": round(r["ms_plain"] / steps, 4),
             "z_start": st0.z, "z_end": st1.z, "attempts": att,
             "plasma_groups_frac": round((st1.plasma_groups - st0.plasma_groups) / groups, 4),
-            "kernel_avg_ms": per}
+            "attempt_path": r["path"], "kernel_avg_ms": per}
 
 
 def main():
@@ -532,10 +554,11 @@ def main():
     steps_per_s = 1000.0 / ms_per_step
 
     field_bytes = 8.0 * nx * nx * nt
-    passes = FIELD_PASSES_SPLIT if (nx, nt) == (128, 1024) else FIELD_PASSES
+    slot_names, passes, groups = attempt_tables(r["path"], nx, nt)
+    att_passes = sum(passes.values())
     kernels = {}
-    for g, slots in GROUPS.items():
-        ms = sum(kms[SLOTS.index(k)] for k in slots)
+    for g, slots in groups.items():
+        ms = sum(kms[slot_names.index(k)] for k in slots)
         launches = len(slots) * args.steps
         byts = passes[slots[0]] * field_bytes
         avg_ms = ms / launches
@@ -553,7 +576,7 @@ def main():
         roofline["traffic_source"] = ("profiles/uppe3d_%dx%dx%d_hbm_traffic.md (rocprofv3 "
                                       "FETCH_SIZE + WRITE_SIZE, copy-calibrated)" % (nx, nx, nt))
     step_gbs_37 = 37 * field_bytes * steps_per_s / 1e9
-    step_gbs_31 = 31 * field_bytes * steps_per_s / 1e9
+    step_gbs_alg = att_passes * field_bytes * steps_per_s / 1e9
 
     cpu = None
     if args.cpu_attempts > 0:
@@ -589,8 +612,11 @@ def main():
                    "accepted": st1.accepted - st0.accepted,
                    "wall_s": round(wall, 4)},
         "roofline": roofline,
-        "step_hbm_GBps": {"model_37F": round(step_gbs_37, 1), "model_31F": round(step_gbs_31, 1),
-                          "frac_37F": round(step_gbs_37 / HBM_PEAK_GBS, 4)},
+        "attempt_path": r["path"],
+        "step_hbm_GBps": {"model_37F": round(step_gbs_37, 1),
+                          "algorithmic": round(step_gbs_alg, 1),
+                          "algorithmic_field_passes": att_passes,
+                          "frac_algorithmic": round(step_gbs_alg / HBM_PEAK_GBS, 4)},
         "kernels": {g: {k: (round(v, 5) if isinstance(v, float) else v) for k, v in kd.items()}
                     for g, kd in kernels.items()},
         "cpu_baseline": cpu,

@@ -144,6 +144,20 @@ int uppe3d_apply_mirror(uppe3d_plan *p, double R, double amp, double x0, double 
 #define UPPE_NORM_L2 1
 int uppe3d_set_error_norm(uppe3d_plan *p, int norm);
 
+/* Attempt dataflow (no counterpart in the reference, whose step is one CuPy expression
+ * sequence, solver_3d.py:100-160).  A plan runs each attempt either as 11 launches over 7
+ * fields (31 field passes: the stage combinations in (kx, ky, omega)) or, by default where it
+ * applies, restructured: every stage field kept in (x, y, omega), the two N evaluations of
+ * stages 2-3 and of stages 4-5 fused into one t-pass each, 6 launch groups over 4 fields
+ * (21 field passes).  Same arithmetic up to FP32 rounding.  The restructured attempt applies
+ * when nx <= 128, nt <= 1024, every filter value is positive, min filter * exp(-max damp * h)
+ * >= 1e-30 for the run's max_step (replayed attempts: their h), and no stage record is set.
+ * uppe3d_set_fused(p, 0) forces the 11-launch attempt; uppe3d_attempt_path reports the path
+ * the next attempt takes (1 restructured, 0 not).  uppe3d_attempt_timed's kernel_ms then holds
+ * T1, XY-A, T23, XY-B, T45, XY-E, controller and four zeros. */
+int uppe3d_set_fused(uppe3d_plan *p, int enable);
+int uppe3d_attempt_path(uppe3d_plan *p, int *fused);
+
 /* Adjoint of the 3+1D step (reverse mode through a frozen step sequence; no counterpart in
  * the reference).  uppe3d_set_stage_record makes every following attempt copy its stage
  * fields X2, X3, X4 (the inputs of the 2nd..4th nonlinear evaluations, same layout as the

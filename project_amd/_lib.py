@@ -94,6 +94,8 @@ SIGNATURES = {
     "uppe3d_apply_mirror": (c_int, [c_void_p, c_double, c_double, c_double, c_double, c_double,
                                     c_double, c_void_p]),
     "uppe3d_set_error_norm": (c_int, [c_void_p, c_int]),
+    "uppe3d_set_fused": (c_int, [c_void_p, c_int]),
+    "uppe3d_attempt_path": (c_int, [c_void_p, P(c_int)]),
     "uppe3d_set_stage_record": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "uppe3d_adjoint_workspace_size": (c_int, [P(Uppe3DDesc), P(c_size_t)]),
     "uppe3d_adjoint_step": (c_int, [c_void_p, c_double] + [c_void_p] * 7),

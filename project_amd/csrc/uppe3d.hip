@@ -552,7 +552,9 @@ struct TStage {
     // shared with the HBM stage (separated by workgroup barriers); that region holds the
     // whole group, so all rows are loaded (and stored) in one chunk.  Otherwise the stage
     // holds CH = 4 chunks of 64 rows, double-buffered in registers.
-    static constexpr bool fast(bool plasma) { return NT == 1024 && LINES == 16 && !plasma; }
+    static constexpr bool fast(bool plasma) {
+        return NT == 1024 && (LINES == 16 || LINES == 8) && !plasma;
+    }
     static constexpr int ch(bool plasma) { return fast(plasma) ? R : (R < 4 ? R : 4); }
     static constexpr int lds(bool plasma) {
         return fast(plasma) && LINES * W1024::REGION > ch(plasma) * LINES * SP
@@ -561,69 +563,19 @@ struct TStage {
     }
 };
 
-template <int NT, int LINES, int MODE, bool PLASMA>
-__device__ __forceinline__ void tpass_group(int blk, float2 *stage, Fields3D f,
-                                            const float2 *in_fixed, int in_is_state,
-                                            float2 *out, const Tabs3D &tb, const Phys3D &ph,
-                                            Ctl3D *ctl, int nx, int ny, int *list, int slot) {
-    constexpr int R = TStage<NT, LINES>::R;
-    constexpr int CH = TStage<NT, LINES>::ch(PLASMA);
-    constexpr int SP = TStage<NT, LINES>::SP;
+// N of one line held by each wavefront, in place: v (omega, layout of the instantiation) ->
+// mask [shock] FFT_t(Kerr [+ Plasma]) of iFFT_t(v).  The lean instantiation (!PLASMA) has no
+// plasma code: it returns false when some line of the workgroup reaches the plasma
+// threshold (uniform over the workgroup; v is then undefined and the caller defers the
+// group).  The PLASMA instantiation evaluates the plasma term on the lines that reach it.
+template <int NT, int LINES, bool PLASMA>
+__device__ __forceinline__ bool line_neval(float2 (&v)[NT / 64], float2 *stage, const Tabs3D &tb,
+                                           const Phys3D &ph, const Ctl3D *ctl) {
+    constexpr int R = NT / 64;
     constexpr bool FAST = TStage<NT, LINES>::fast(PLASMA);
-    const int cur = ctl->cur;
-    const float2 *in = in_is_state ? (cur ? f.B1 : f.B0) : in_fixed;
-    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int rl = __brev(lane) >> 26;
-    // lane's first frequency in the register layout of the line transform
     const int sl = FAST ? lane : rl;
-    const int bpx = ny / LINES;
-    const int x = blk / bpx;
-    const int y0 = (blk % bpx) * LINES;
-    const int plane = nx * ny;
-    const int q = tid % LINES, row = tid / LINES;
-    // 32-bit element offsets from a uniform base (fields hold < 2^28 elements)
-    const int col0 = x * ny + y0;
-    const int lo = row * plane + col0 + q;
-
-    float2 v[R];
-    {
-        // HBM -> registers in chunks of CH rows-of-64, next chunk prefetched during the
-        // LDS transposition of the current one
-        float absorb = 1.0f;
-        float2 *bnew = nullptr;
-        if (MODE != 0) {
-            absorb = expf(-tb.damp[col0 + q] * (float)ctl->h);
-            bnew = cur ? f.B0 : f.B1;
-        }
-        float2 g[CH], gn[CH];
-#pragma unroll
-        for (int cc = 0; cc < CH; ++cc) g[cc] = in[lo + 64 * cc * plane];
-#pragma unroll
-        for (int c0 = 0; c0 < R; c0 += CH) {
-            if (c0 + CH < R) {
-#pragma unroll
-                for (int cc = 0; cc < CH; ++cc)
-                    gn[cc] = in[lo + 64 * (c0 + CH + cc) * plane];
-            }
-            if (MODE != 0) {
-#pragma unroll
-                for (int cc = 0; cc < CH; ++cc) {
-                    const int w = row + 64 * (c0 + cc);
-                    bnew[lo + 64 * (c0 + cc) * plane] = cscale(g[cc], tb.filt_w[w] * absorb);
-                }
-            }
-            const int sw = opaque_i(q * SP + row);
-#pragma unroll
-            for (int cc = 0; cc < CH; ++cc) stage[sw + cc * LINES * SP] = g[cc];
-            __syncthreads();
-            const int sr = opaque_i(wv * SP + sl);
-#pragma unroll
-            for (int cc = 0; cc < CH; ++cc) v[c0 + cc] = stage[sr + cc * LINES * SP];
-            __syncthreads();
-#pragma unroll
-            for (int cc = 0; cc < CH; ++cc) g[cc] = gn[cc];
-        }
-    }
     if constexpr (FAST) {
         w1024_inverse(*reinterpret_cast<float2(*)[16]>(&v[0]), stage + wv * W1024::REGION,
                       lane, tb.twA, tb.twB);
@@ -633,11 +585,7 @@ __device__ __forceinline__ void tpass_group(int blk, float2 *stage, Fields3D f,
 
     bool plasma = line_imax<R>(v, ph) >= ctl->plasma_thr;
     if (!PLASMA) {
-        const int defer = __syncthreads_or(plasma ? 1 : 0);
-        if (defer) {
-            if (tid == 0) list[atomicAdd(&ctl->plasma_groups[slot], 1)] = blk;
-            return;
-        }
+        if (__syncthreads_or(plasma ? 1 : 0)) return false;
         plasma = false;
     }
 
@@ -702,8 +650,69 @@ __device__ __forceinline__ void tpass_group(int blk, float2 *stage, Fields3D f,
             for (int r = 0; r < R; ++r) v[r] = caxpy(v[r], tb.mask_w[sl + 64 * r], p[r]);
         }
     }
+    return true;
+}
 
-    if constexpr (FAST) __syncthreads();   // every wave is done with its transform region
+// HBM rows of the group (LINES consecutive y, 8 * LINES bytes per omega row) -> one line per
+// wavefront in the register layout of the instantiation, in chunks of CH rows-of-64 through
+// the LDS stage, the next chunk prefetched during the transposition of the current one.
+// lo: offset of the thread's first element (row * plane + first column + q).  With a
+// non-null bnew (final stage), bnew = src * filter(omega) * absorb is written on the way.
+template <int NT, int LINES, bool PLASMA>
+__device__ __forceinline__ void group_load(float2 (&v)[NT / 64], const float2 *__restrict__ in,
+                                           float2 *stage, int lo, int plane, float2 *bnew,
+                                           const float *filt, float absorb) {
+    constexpr int R = NT / 64;
+    constexpr int CH = TStage<NT, LINES>::ch(PLASMA);
+    constexpr int SP = TStage<NT, LINES>::SP;
+    constexpr bool FAST = TStage<NT, LINES>::fast(PLASMA);
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int sl = FAST ? lane : (__brev(lane) >> 26);
+    const int q = tid % LINES, row = tid / LINES;
+    float2 g[CH], gn[CH];
+#pragma unroll
+    for (int cc = 0; cc < CH; ++cc) g[cc] = in[lo + 64 * cc * plane];
+#pragma unroll
+    for (int c0 = 0; c0 < R; c0 += CH) {
+        if (c0 + CH < R) {
+#pragma unroll
+            for (int cc = 0; cc < CH; ++cc) gn[cc] = in[lo + 64 * (c0 + CH + cc) * plane];
+        }
+        if (bnew != nullptr) {
+#pragma unroll
+            for (int cc = 0; cc < CH; ++cc) {
+                const int w = row + 64 * (c0 + cc);
+                bnew[lo + 64 * (c0 + cc) * plane] = cscale(g[cc], filt[w] * absorb);
+            }
+        }
+        const int sw = opaque_i(q * SP + row);
+#pragma unroll
+        for (int cc = 0; cc < CH; ++cc) stage[sw + cc * LINES * SP] = g[cc];
+        __syncthreads();
+        const int sr = opaque_i(wv * SP + sl);
+#pragma unroll
+        for (int cc = 0; cc < CH; ++cc) v[c0 + cc] = stage[sr + cc * LINES * SP];
+        __syncthreads();
+#pragma unroll
+        for (int cc = 0; cc < CH; ++cc) g[cc] = gn[cc];
+    }
+}
+
+// The inverse of group_load: one line per wavefront -> HBM rows of the group.  SUB: the row
+// written is sub - v (sub read at the same element) instead of v.  Every wavefront must be
+// done with its transform region (the FAST stage aliases them): a barrier comes first.
+template <int NT, int LINES, bool PLASMA, bool SUB>
+__device__ __forceinline__ void group_store(const float2 (&v)[NT / 64], float2 *out,
+                                            const float2 *sub, float2 *stage, int lo,
+                                            int plane) {
+    constexpr int R = NT / 64;
+    constexpr int CH = TStage<NT, LINES>::ch(PLASMA);
+    constexpr int SP = TStage<NT, LINES>::SP;
+    constexpr bool FAST = TStage<NT, LINES>::fast(PLASMA);
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int sl = FAST ? lane : (__brev(lane) >> 26);
+    const int q = tid % LINES, row = tid / LINES;
+    if constexpr (FAST) __syncthreads();
 #pragma unroll
     for (int c0 = 0; c0 < R; c0 += CH) {
         const int sw = opaque_i(wv * SP + sl);
@@ -715,11 +724,132 @@ __device__ __forceinline__ void tpass_group(int blk, float2 *stage, Fields3D f,
         for (int cc = 0; cc < CH; ++cc) {
             const int g = lo + 64 * (c0 + cc) * plane;
             float2 val = stage[sr + cc * LINES * SP];
-            if (MODE == 1) val = csub(f.Y4[g], val);
+            if (SUB) val = csub(sub[g], val);
             out[g] = val;
         }
         __syncthreads();
     }
+}
+
+template <int NT, int LINES, int MODE, bool PLASMA>
+__device__ __forceinline__ void tpass_group(int blk, float2 *stage, Fields3D f,
+                                            const float2 *in_fixed, int in_is_state,
+                                            float2 *out, const Tabs3D &tb, const Phys3D &ph,
+                                            Ctl3D *ctl, int nx, int ny, int *list, int slot) {
+    constexpr int R = TStage<NT, LINES>::R;
+    const int cur = ctl->cur;
+    const float2 *in = in_is_state ? (cur ? f.B1 : f.B0) : in_fixed;
+    const int tid = threadIdx.x;
+    const int bpx = ny / LINES;
+    const int x = blk / bpx;
+    const int y0 = (blk % bpx) * LINES;
+    const int plane = nx * ny;
+    const int q = tid % LINES, row = tid / LINES;
+    // 32-bit element offsets from a uniform base (fields hold < 2^28 elements)
+    const int col0 = x * ny + y0;
+    const int lo = row * plane + col0 + q;
+
+    float2 v[R];
+    float absorb = 1.0f;
+    float2 *bnew = nullptr;
+    if (MODE != 0) {
+        absorb = expf(-tb.damp[col0 + q] * (float)ctl->h);
+        bnew = cur ? f.B0 : f.B1;
+    }
+    group_load<NT, LINES, PLASMA>(v, in, stage, lo, plane, bnew, tb.filt_w, absorb);
+    if (!line_neval<NT, LINES, PLASMA>(v, stage, tb, ph, ctl)) {
+        if (tid == 0) list[atomicAdd(&ctl->plasma_groups[slot], 1)] = blk;
+        return;
+    }
+    group_store<NT, LINES, PLASMA, MODE == 1>(v, out, f.Y4, stage, lo, plane);
+}
+
+// Fused t-passes of the restructured attempt (every field in the M domain (x, y, omega); the
+// xy passes apply E = FFT2^-1 exp(h/2 D) FFT2):
+//   MODE 3 (stages 2 and 3), in place over f1 = X2 = E (B + h/2 Y1), f2 = Zh = h/6 E Y1:
+//     Y2 = N(X2), X3 = X2 - 3 Zh + h/2 Y2 (= E B + h/2 Y2), Y3 = N(X3);
+//     f1 <- h Y3,  f2 <- Zh + h/3 (Y2 + Y3)
+//   MODE 4 (stages 4 and 5), over f1 = X4 = E (E B + h Y3), f2 = R' = E (E B + Zh + h/3 (Y2 + Y3)):
+//     Y4 = N(X4), a4 = R' + h/6 Y4 (the step's result), B' = a4 filter absorb -> spare state,
+//     Y5 = N(a4);  f1 <- D = Y4 - Y5
+// Only increments leave MODE 3: the xy pass that follows forms E E B from the state itself, so
+// the state crosses one FFT2 round trip per attempt, as in the 11-launch attempt.
+// Nothing is written to f1 or f2 before both N evaluations have passed the plasma check, so
+// a deferred group is re-run from intact inputs by the PLASMA instantiation.
+template <int NT, int LINES, int MODE, bool PLASMA>
+__device__ __forceinline__ void tfused_group(int blk, float2 *stage, Fields3D f, const Tabs3D &tb,
+                                             const Phys3D &ph, Ctl3D *ctl, int nx, int ny,
+                                             int *list, int slot) {
+    constexpr int R = TStage<NT, LINES>::R;
+    constexpr bool FAST = TStage<NT, LINES>::fast(PLASMA);
+    const int tid = threadIdx.x;
+    const int bpx = ny / LINES;
+    const int x = blk / bpx;
+    const int y0 = (blk % bpx) * LINES;
+    const int plane = nx * ny;
+    const int q = tid % LINES, row = tid / LINES;
+    const int col0 = x * ny + y0;
+    const int lo = row * plane + col0 + q;
+    const float h = (float)ctl->h;
+    float2 *const f1 = f.X, *const f2 = f.Yw;
+    float2 a[R], b[R];
+    group_load<NT, LINES, PLASMA>(a, f1, stage, lo, plane, nullptr, nullptr, 1.0f);
+    bool ok;
+    if constexpr (MODE == 3) {
+        float2 v[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) v[k] = a[k];
+        ok = line_neval<NT, LINES, PLASMA>(v, stage, tb, ph, ctl);   // v = Y2
+        if (ok) {
+            if constexpr (FAST) __syncthreads();   // stage free again
+            group_load<NT, LINES, PLASMA>(b, f2, stage, lo, plane, nullptr, nullptr, 1.0f);
+            const float h2 = 0.5f * h, h3 = h / 3.0f;
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                a[k] = caxpy(caxpy(a[k], -3.0f, b[k]), h2, v[k]);   // X3
+                b[k] = caxpy(b[k], h3, v[k]);
+            }
+        }
+        if (ok) ok = line_neval<NT, LINES, PLASMA>(a, stage, tb, ph, ctl);   // a = Y3
+        if (ok) {
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                b[k] = caxpy(b[k], h / 3.0f, a[k]);
+                a[k] = cscale(a[k], h);
+            }
+            group_store<NT, LINES, PLASMA, false>(a, f1, nullptr, stage, lo, plane);
+            group_store<NT, LINES, PLASMA, false>(b, f2, nullptr, stage, lo, plane);
+        }
+    } else {
+        ok = line_neval<NT, LINES, PLASMA>(a, stage, tb, ph, ctl);   // a = Y4
+        if (ok) {
+            if constexpr (FAST) __syncthreads();
+            group_load<NT, LINES, PLASMA>(b, f2, stage, lo, plane, nullptr, nullptr, 1.0f);
+            const float h6 = h / 6.0f;
+#pragma unroll
+            for (int k = 0; k < R; ++k) b[k] = caxpy(b[k], h6, a[k]);   // a4
+            {
+                // B' = a4 filter(omega) absorb(x, y) into the spare state (wavefront wv holds
+                // the line of column col0 + wv)
+                float2 v[R];
+                const int wv = tid >> 6, lane = tid & 63;
+                const int sl = FAST ? lane : (__brev(lane) >> 26);
+                const float absorb = expf(-tb.damp[col0 + wv] * h);
+                const float *filt = opaque(tb.filt_w);
+#pragma unroll
+                for (int k = 0; k < R; ++k) v[k] = cscale(b[k], filt[sl + 64 * k] * absorb);
+                group_store<NT, LINES, PLASMA, false>(v, ctl->cur ? f.B0 : f.B1, nullptr, stage,
+                                                      lo, plane);
+            }
+            ok = line_neval<NT, LINES, PLASMA>(b, stage, tb, ph, ctl);   // b = Y5
+            if (ok) {
+#pragma unroll
+                for (int k = 0; k < R; ++k) a[k] = csub(a[k], b[k]);   // D = Y4 - Y5
+                group_store<NT, LINES, PLASMA, false>(a, f1, nullptr, stage, lo, plane);
+            }
+        }
+    }
+    if (!ok && !PLASMA && tid == 0) list[atomicAdd(&ctl->plasma_groups[slot], 1)] = blk;
 }
 
 // t-pass launch pair.  PLASMA = false runs every workgroup of lines; PLASMA = true runs the
@@ -751,6 +881,24 @@ __global__ __launch_bounds__(64 * LINES) void tpass_kernel(Fields3D f, const flo
         for (int i = blockIdx.x; i < n; i += gridDim.x)
             tpass_group<NT, LINES, MODE, true>(list[i], stage, f, in_fixed, in_is_state, out,
                                                tb, ph, ctl, nx, ny, list, slot);
+    }
+}
+
+// Fused t-pass launch pair (MODE 3 or 4 of tfused_group), as tpass_kernel.
+template <int NT, int LINES, int MODE, bool PLASMA>
+__global__ __launch_bounds__(64 * LINES) void tfused_kernel(Fields3D f, Tabs3D tb, Phys3D ph,
+                                                            Ctl3D *ctl, int nx, int ny,
+                                                            int *list, int slot) {
+    __shared__ float2 stage[TStage<NT, LINES>::lds(PLASMA)];
+    if (ctl->done) return;
+    if (!PLASMA) {
+        const int g = LINES == 8 ? xcd_pair_group(blockIdx.x, gridDim.x) : blockIdx.x;
+        tfused_group<NT, LINES, MODE, false>(g, stage, f, tb, ph, ctl, nx, ny, list, slot);
+    } else {
+        const int n = ctl->plasma_groups[slot];
+        for (int i = blockIdx.x; i < n; i += gridDim.x)
+            tfused_group<NT, LINES, MODE, true>(list[i], stage, f, tb, ph, ctl, nx, ny, list,
+                                                slot);
     }
 }
 
@@ -1347,6 +1495,146 @@ __global__ __launch_bounds__(xy_threads(NX, NY)) void xypass_kernel(Fields3D f, 
     }
 }
 
+// plane of src -> LDS -> FFT2 (forward, in LDS; ends with a barrier)
+template <int NX, int NY>
+__device__ __forceinline__ void xy_load_fft(float2 *lds, const float2 *__restrict__ src, int tid,
+                                            const Tabs3D &tb) {
+    using C = XY<NX, NY>;
+#pragma unroll
+    for (int i = 0; i < C::EPT; ++i) {
+        const int e = tid + C::T * i;
+        lds[lslot<NY>(e)] = src[e];
+    }
+    __syncthreads();
+    fft2_lds<NX, NY, 1>(lds, tid, tb);
+}
+
+// the plane in LDS (complete) -> inverse FFT2 / (NX NY) -> dst; LDS free again at the end
+template <int NX, int NY>
+__device__ __forceinline__ void xy_ifft_store(float2 *lds, float2 *__restrict__ dst, int tid,
+                                              float inv, const Tabs3D &tb) {
+    using C = XY<NX, NY>;
+    __syncthreads();
+    fft2_lds<NX, NY, -1>(lds, tid, tb);
+#pragma unroll
+    for (int i = 0; i < C::EPT; ++i) {
+        const int e = tid + C::T * i;
+        dst[e] = cscale(lds[lslot<NY>(e)], inv);
+    }
+    __syncthreads();
+}
+
+// xy passes of the restructured attempt (fields in the M domain (x, y, omega); E = exp(h/2 D),
+// F = FFT2):
+// MODE 8: X <- X2 = F^-1 E (F(B) + h/2 F(Y1)),  Yw <- Zh = F^-1 h/6 E F(Y1)   (Yw holds Y1)
+// MODE 9: c = E F(B);  X <- X4 = F^-1 E (c + F(X)),  Yw <- R' = F^-1 E (c + F(Yw))
+//         (X, Yw hold the increments h Y3 and Zh + h/3 (Y2 + Y3) of the fused t-pass)
+// MODE 10: the error norms: A4 = FFT2(a4) with a4 = B' / (filter absorb) (B' the spare state
+// written by the last fused t-pass), max |A4|^2 (and the sum) and max |FFT2(X = D)|^2.
+// The state enters MODE 8 and MODE 9 and leaves through one inverse transform (X4, R'), so it
+// crosses one FFT2 round trip per attempt.  Each plane is read completely before it is
+// written, so the passes work in place.
+template <int NX, int NY, int MODE>
+__global__ __launch_bounds__(xy_threads(NX, NY)) void xyf_kernel(Fields3D f, Tabs3D tb,
+                                                            Ctl3D *ctl) {
+    using C = XY<NX, NY>;
+    __shared__ float2 lds[NX * C::LD];
+    __shared__ unsigned int red[C::T / 64 > 0 ? C::T / 64 : 1];
+    __shared__ double reds[C::T / 64 > 0 ? C::T / 64 : 1];
+    if (ctl->done) return;
+    const int w = blockIdx.x;
+    const int tid = threadIdx.x;
+    const size_t pb = (size_t)w * C::P;
+    const float h = (float)ctl->h;
+    const float inv = 1.0f / (float)(NX * NY);
+    float2 *const Xp = f.X + pb, *const Yp = f.Yw + pb;
+    if (MODE == 10) {
+        const float2 *Bn = (ctl->cur ? f.B0 : f.B1) + pb;
+        const float fi = 1.0f / tb.filt_w[w];
+        const float fi2 = fi * fi;
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) {
+            const int e = tid + C::T * i;
+            lds[lslot<NY>(e)] = cscale(Bn[e], expf(tb.damp[e] * h));
+        }
+        __syncthreads();
+        fft2_lds<NX, NY, 1>(lds, tid, tb);
+        unsigned int m = 0;
+        float sq = 0.0f;
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) {
+            const float a2 = cabs2(lds[lslot<NY>(tid + C::T * i)]) * fi2;
+            const unsigned int v = __float_as_uint(a2);
+            m = v > m ? v : m;
+            sq += a2;
+        }
+        block_max_to<C::T>(m, &ctl->maxA_bits, red, tid);
+        if (ctl->l2) block_sum_to<C::T>(sq, &ctl->sumA, reds, tid);
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) {
+            const int e = tid + C::T * i;
+            lds[lslot<NY>(e)] = Xp[e];
+        }
+        __syncthreads();
+        fft2_lds<NX, NY, 1>(lds, tid, tb);
+        m = 0;
+        sq = 0.0f;
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) {
+            const float a2 = cabs2(lds[lslot<NY>(tid + C::T * i)]);
+            const unsigned int v = __float_as_uint(a2);
+            m = v > m ? v : m;
+            sq += a2;
+        }
+        __syncthreads();   // red / reds of the first reduction have been read
+        block_max_to<C::T>(m, &ctl->maxD_bits, red, tid);
+        if (ctl->l2) block_sum_to<C::T>(sq, &ctl->sumD, reds, tid);
+        return;
+    }
+    const float2 ed = tb.ph_d[w], eref = tb.ph_r[w];
+    const float2 *B = (ctl->cur ? f.B1 : f.B0) + pb;
+    float2 b[C::EPT], E[C::EPT];
+#pragma unroll
+    for (int i = 0; i < C::EPT; ++i) E[i] = eprop<NY>(tid + C::T * i, w, 0.5f * h, tb, ed, eref);
+    if (MODE == 8) {
+        xy_load_fft<NX, NY>(lds, Yp, tid, tb);
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) b[i] = lds[lslot<NY>(tid + C::T * i)];
+        __syncthreads();
+        xy_load_fft<NX, NY>(lds, B, tid, tb);
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) {
+            const int e = tid + C::T * i;
+            lds[lslot<NY>(e)] = cmul(E[i], caxpy(lds[lslot<NY>(e)], 0.5f * h, b[i]));   // X2
+            b[i] = cmul(E[i], cscale(b[i], h / 6.0f));                                    // Zh
+        }
+        xy_ifft_store<NX, NY>(lds, Xp, tid, inv, tb);
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) lds[lslot<NY>(tid + C::T * i)] = b[i];
+        xy_ifft_store<NX, NY>(lds, Yp, tid, inv, tb);
+    } else {
+        xy_load_fft<NX, NY>(lds, B, tid, tb);
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) b[i] = cmul(E[i], lds[lslot<NY>(tid + C::T * i)]);
+        __syncthreads();
+        xy_load_fft<NX, NY>(lds, Xp, tid, tb);
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) {
+            const int e = tid + C::T * i;
+            lds[lslot<NY>(e)] = cmul(E[i], cadd(b[i], lds[lslot<NY>(e)]));   // X4
+        }
+        xy_ifft_store<NX, NY>(lds, Xp, tid, inv, tb);
+        xy_load_fft<NX, NY>(lds, Yp, tid, tb);
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) {
+            const int e = tid + C::T * i;
+            lds[lslot<NY>(e)] = cmul(E[i], cadd(b[i], lds[lslot<NY>(e)]));   // R'
+        }
+        xy_ifft_store<NX, NY>(lds, Yp, tid, inv, tb);
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // 128 x 128 planes.  Each length-128 transform (a row along y, or a column along x) is held
 // by 8 adjacent lanes, 16 values per lane, as two register DFT passes (16 x 8):
@@ -1463,6 +1751,22 @@ __device__ __forceinline__ void fft2_128(float2 (&v)[16], const float2 *__restri
     fwd128(v, lds + wv * x128::WREG, lane, tw);
 }
 
+// as fft2_128, of src * exp(damp h) (damp per plane element)
+__device__ __forceinline__ void fft2_128_unabsorb(float2 (&v)[16], const float2 *__restrict__ src,
+                                                  const float *__restrict__ damp, float h,
+                                                  float2 *lds, int wv, int lane,
+                                                  const float2 *tw) {
+    const int row = 8 * wv + (lane >> 3), j = lane & 7;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int e = row * 128 + j + 8 * r;
+        v[r] = cscale(src[e], expf(damp[e] * h));
+    }
+    fwd128(v, lds + wv * x128::WREG, lane, tw);
+    rows_to_cols(v, lds, wv, lane);
+    fwd128(v, lds + wv * x128::WREG, lane, tw);
+}
+
 // FFT2 of the difference srcA - srcB of two planes (row-major, HBM)
 __device__ __forceinline__ void fft2_128_diff(float2 (&v)[16], const float2 *__restrict__ srcA,
                                               const float2 *__restrict__ srcB, float2 *lds,
@@ -1566,6 +1870,89 @@ __global__ __launch_bounds__(x128::T) void xy128_kernel(Fields3D f, Tabs3D tb, C
         if (ctl->l2) block_sum_to<x128::T>(sq, &ctl->sumA, reds, tid);
     }
     ifft2_128(a, Xp, lds, wv, lane, tb.tw128);
+}
+
+// The xy passes of xyf_kernel (MODE 8, 9, 10) for 128 x 128 planes, the plane in registers
+// (16 values per thread) between the transforms.
+template <int MODE>
+__global__ __launch_bounds__(x128::T) void xy128f_kernel(Fields3D f, Tabs3D tb, Ctl3D *ctl) {
+    __shared__ float2 lds[x128::LDS];
+    __shared__ unsigned int red[x128::T / 64];
+    __shared__ double reds[x128::T / 64];
+    if (ctl->done) return;
+    const int w = blockIdx.x;
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const size_t pb = (size_t)w * 16384;
+    const float h = (float)ctl->h;
+    float2 *const Xp = f.X + pb, *const Yp = f.Yw + pb;
+    float2 a[16];
+    if (MODE == 10) {
+        const float2 *Bn = (ctl->cur ? f.B0 : f.B1) + pb;
+        const float fi = 1.0f / tb.filt_w[w];
+        const float fi2 = fi * fi;
+        fft2_128_unabsorb(a, Bn, tb.damp, h, lds, wv, lane, tb.tw128);
+        unsigned int m = 0;
+        float sq = 0.0f;
+#pragma unroll
+        for (int n = 0; n < 16; ++n) {
+            const float a2 = cabs2(a[n]) * fi2;
+            const unsigned int u = __float_as_uint(a2);
+            m = u > m ? u : m;
+            sq += a2;
+        }
+        block_max_to<x128::T>(m, &ctl->maxA_bits, red, tid);
+        if (ctl->l2) block_sum_to<x128::T>(sq, &ctl->sumA, reds, tid);
+        fft2_128(a, Xp, lds, wv, lane, tb.tw128);   // barriers inside: red is free again
+        m = 0;
+        sq = 0.0f;
+#pragma unroll
+        for (int n = 0; n < 16; ++n) {
+            const float a2 = cabs2(a[n]);
+            const unsigned int u = __float_as_uint(a2);
+            m = u > m ? u : m;
+            sq += a2;
+        }
+        block_max_to<x128::T>(m, &ctl->maxD_bits, red, tid);
+        if (ctl->l2) block_sum_to<x128::T>(sq, &ctl->sumD, reds, tid);
+        return;
+    }
+    const float2 ed = tb.ph_d[w], eref = tb.ph_r[w];
+    const int ky = 8 * wv + (lane >> 3), kxb = 2 * (lane & 7);
+    const float kyy = tb.ky2[ky];
+    const float2 *B = (ctl->cur ? f.B1 : f.B0) + pb;
+    float2 b[16];
+    if (MODE == 8) {
+        fft2_128(b, Yp, lds, wv, lane, tb.tw128);
+        fft2_128(a, B, lds, wv, lane, tb.tw128);
+#pragma unroll
+        for (int n = 0; n < 16; ++n) {
+            const int kx = kxb + (n >> 3) + 16 * (n & 7);
+            const float2 E = eprop_k(tb.kx2[kx] + kyy, w, 0.5f * h, tb, ed, eref);
+            a[n] = cmul(E, caxpy(a[n], 0.5f * h, b[n]));   // X2
+            b[n] = cmul(E, cscale(b[n], h / 6.0f));        // Zh
+            __builtin_amdgcn_sched_barrier(0);   // one element at a time: bounded registers
+        }
+        ifft2_128(a, Xp, lds, wv, lane, tb.tw128);
+        ifft2_128(b, Yp, lds, wv, lane, tb.tw128);
+    } else {
+        float2 E[16];
+        fft2_128(a, B, lds, wv, lane, tb.tw128);
+#pragma unroll
+        for (int n = 0; n < 16; ++n) {
+            const int kx = kxb + (n >> 3) + 16 * (n & 7);
+            E[n] = eprop_k(tb.kx2[kx] + kyy, w, 0.5f * h, tb, ed, eref);
+            a[n] = cmul(E[n], a[n]);   // E F(B)
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        fft2_128(b, Xp, lds, wv, lane, tb.tw128);
+#pragma unroll
+        for (int n = 0; n < 16; ++n) b[n] = cmul(E[n], cadd(a[n], b[n]));   // X4
+        ifft2_128(b, Xp, lds, wv, lane, tb.tw128);
+        fft2_128(b, Yp, lds, wv, lane, tb.tw128);
+#pragma unroll
+        for (int n = 0; n < 16; ++n) b[n] = cmul(E[n], cadd(a[n], b[n]));   // R'
+        ifft2_128(b, Yp, lds, wv, lane, tb.tw128);
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -2233,6 +2620,15 @@ struct uppe3d_plan {
     int plasma_small;   // 1: the last uppe3d_run window deferred nothing to the plasma launch
     size_t ws_bytes;
     void *rec[3]; // optional copies of the stage fields X2, X3, X4 of every attempt
+    // restructured attempt (enqueue_attempt_fused): allowed for planes held by one workgroup
+    // (nx <= 128), N_t <= 1024 and every filter value positive; switched per step size so that the
+    // error pass can divide the filter and absorber out of B' (min over the grid of
+    // filter * exp(-damp h) at least 1e-30: FP32 keeps full precision for every element
+    // above 1e-8 of the field maximum)
+    int fused_enabled;  // uppe3d_set_fused
+    int fused_ok;       // grid and filter admit the restructured attempt
+    int step_ok;        // min filter * exp(-max damp * h_max) >= 1e-30 for the current run
+    double filt_min, damp_max;
 };
 
 namespace {
@@ -2457,9 +2853,128 @@ int record_stage(uppe3d_plan *p, int k, hipStream_t s) {
     return UPPE_OK;
 }
 
+#define LAUNCH_F(NT, LINES)                                                                    \
+    case NT:                                                                                   \
+        if (mode == 3) {                                                                       \
+            tfused_kernel<NT, LINES, 3, false><<<grid, 64 * LINES, 0, s>>>(                    \
+                p->f, p->tb, p->ph, p->ctl, nx, ny, p->list, slot);                            \
+            tfused_kernel<NT, LINES, 3, true><<<pgrid, 64 * LINES, 0, s>>>(                    \
+                p->f, p->tb, p->ph, p->ctl, nx, ny, p->list, slot);                            \
+        } else {                                                                               \
+            tfused_kernel<NT, LINES, 4, false><<<grid, 64 * LINES, 0, s>>>(                    \
+                p->f, p->tb, p->ph, p->ctl, nx, ny, p->list, slot);                            \
+            tfused_kernel<NT, LINES, 4, true><<<pgrid, 64 * LINES, 0, s>>>(                    \
+                p->f, p->tb, p->ph, p->ctl, nx, ny, p->list, slot);                            \
+        }                                                                                      \
+        break;
+
+// fused t-pass (mode 3: stages 2-3, mode 4: stages 4-5) on the fields X, Yw in place
+int launch_tfused(uppe3d_plan *p, int mode, int slot, hipStream_t s) {
+    const int nx = p->d.nx, ny = p->d.ny;
+    const int lines = p->d.nt >= 1024 ? 8 : 16;
+    const int groups = nx * (ny / lines);
+    const int pg = p->plasma_small ? 8 : p->ncu;
+    const dim3 grid(groups), pgrid(groups < pg ? groups : pg);
+    switch (p->d.nt) {
+        LAUNCH_F(128, 16)
+        LAUNCH_F(256, 16)
+        LAUNCH_F(512, 16)
+        LAUNCH_F(1024, 8)
+        default:
+            return UPPE_EINVAL;
+    }
+    UPPE_HIP(hipGetLastError());
+    return UPPE_OK;
+}
+#undef LAUNCH_F
+
+template <int N>
+int launch_xyf_n(uppe3d_plan *p, int mode, hipStream_t s) {
+    const dim3 grid(p->d.nt), block(XY<N, N>::T);
+    switch (mode) {
+        case 8: xyf_kernel<N, N, 8><<<grid, block, 0, s>>>(p->f, p->tb, p->ctl); break;
+        case 9: xyf_kernel<N, N, 9><<<grid, block, 0, s>>>(p->f, p->tb, p->ctl); break;
+        default: xyf_kernel<N, N, 10><<<grid, block, 0, s>>>(p->f, p->tb, p->ctl); break;
+    }
+    UPPE_HIP(hipGetLastError());
+    return UPPE_OK;
+}
+
+int launch_xyf(uppe3d_plan *p, int mode, hipStream_t s) {
+    switch (p->d.nx) {
+        case 16: return launch_xyf_n<16>(p, mode, s);
+        case 32: return launch_xyf_n<32>(p, mode, s);
+        case 64: return launch_xyf_n<64>(p, mode, s);
+        case 128: {
+            const dim3 grid(p->d.nt), block(x128::T);
+            switch (mode) {
+                case 8: xy128f_kernel<8><<<grid, block, 0, s>>>(p->f, p->tb, p->ctl); break;
+                case 9: xy128f_kernel<9><<<grid, block, 0, s>>>(p->f, p->tb, p->ctl); break;
+                default: xy128f_kernel<10><<<grid, block, 0, s>>>(p->f, p->tb, p->ctl); break;
+            }
+            UPPE_HIP(hipGetLastError());
+            return UPPE_OK;
+        }
+    }
+    return UPPE_EINVAL;
+}
+
+bool use_fused(const uppe3d_plan *p) {
+    return p->fused_enabled && p->fused_ok && p->step_ok && p->rec[0] == nullptr &&
+           p->rec[1] == nullptr && p->rec[2] == nullptr;
+}
+
+// whether the restructured attempt may divide filter * absorber out of B' for steps <= hmax
+int fused_step_ok(const uppe3d_plan *p, double hmax) {
+    return p->filt_min * exp(-p->damp_max * hmax) >= 1e-30 ? 1 : 0;
+}
+
+// The restructured attempt: 6 launch groups on 4 fields, 21 field passes
+// (T1 2, XY-A 4, T23 4, XY-B 5, T45 4, XY-E 2), all fields in the M domain (x, y, omega):
+//   T1:   Yw <- Y1 = N(B)
+//   XY-A: X <- X2 = E (B + h/2 Y1),  Yw <- Zh = h/6 E Y1                (E = F^-1 e^{hD/2} F)
+//   T23:  Y2 = N(X2), X3 = X2 - 3 Zh + h/2 Y2, Y3 = N(X3):  X <- h Y3, Yw <- Zh + h/3 (Y2 + Y3)
+//   XY-B: X <- X4 = E (E B + h Y3),  Yw <- R' = E (E B + Zh + h/3 (Y2 + Y3))
+//   T45:  Y4 = N(X4), a4 = R' + h/6 Y4, B' = a4 filter absorb (spare state), Y5 = N(a4):
+//         X <- D = Y4 - Y5
+//   XY-E: max |FFT2(a4)|, max |FFT2(D)| (a4 = B' / (filter absorb))
+// with the reference's stage values: X2, X3, X4, a4 are F^-1 of v_ip + h/2 alpha1,
+// v_ip + h/2 alpha2, E(v_ip + h alpha3), A4_kw of solver_3d.py:134-145 (v_ip = E A), R' =
+// F^-1 beta, and A4 - A3 = (h/10) F(D).  The state crosses one FFT2 round trip per attempt
+// (XY-B forms E E F(B) from it), as in the 11-launch attempt.  Slot events as
+// enqueue_attempt's: 7 groups (T1, XY-A, T23, XY-B, T45, XY-E, controller), then 4 empty
+// intervals.
+int enqueue_attempt_fused(uppe3d_plan *p, hipStream_t s, hipEvent_t *ev) {
+    Fields3D &f = p->f;
+    int k = 0;
+    auto mark = [&]() -> int {
+        if (ev) UPPE_HIP(hipEventRecord(ev[k++], s));
+        return UPPE_OK;
+    };
+    RET(mark());
+    RET(launch_tpass(p, 0, 0, nullptr, 1, f.Yw, s));   // T1 on the state
+    RET(mark());
+    RET(launch_xyf(p, 8, s));
+    RET(mark());
+    RET(launch_tfused(p, 3, 1, s));
+    RET(mark());
+    RET(launch_xyf(p, 9, s));
+    RET(mark());
+    RET(launch_tfused(p, 4, 2, s));
+    RET(mark());
+    RET(launch_xyf(p, 10, s));
+    RET(mark());
+    ctl_phase_kernel<<<1, 1024, 0, s>>>(p->ctl, p->tb, p->ph, p->d.nt);
+    UPPE_HIP(hipGetLastError());
+    RET(mark());
+    for (int i = 0; i < 4; ++i) RET(mark());
+    return UPPE_OK;
+}
+
 // One attempt: 11 launches.  With `ev` (12 events) an event is recorded before the first
 // and after every launch, so the caller can read per-kernel durations.
 int enqueue_attempt(uppe3d_plan *p, hipStream_t s, hipEvent_t *ev = nullptr) {
+    if (use_fused(p)) return enqueue_attempt_fused(p, s, ev);
     Fields3D &f = p->f;
     int k = 0;
     auto mark = [&]() -> int {
@@ -2597,6 +3112,19 @@ int uppe3d_plan_create(const uppe3d_desc *d, void *workspace, size_t workspace_b
     p->list = (int *)(ws + L.list);
     p->rec[0] = p->rec[1] = p->rec[2] = nullptr;
     p->plasma_small = 0;
+    {
+        double fmin = 1e300, dmax = 0.0;
+        for (int i = 0; i < nt; ++i) fmin = filt_w[i] < fmin ? filt_w[i] : fmin;
+        for (size_t i = 0; i < (size_t)nx * ny; ++i) dmax = damp[i] > dmax ? damp[i] : dmax;
+        // the values the kernels use are the float32 roundings
+        p->filt_min = (double)(float)fmin;
+        p->damp_max = (double)(float)dmax;
+        // N_t 2048 keeps the classic attempt: three line arrays of 32 values per lane exceed
+        // the register file of a fused t-pass
+        p->fused_ok = (nx <= 128 && nt <= 1024 && p->filt_min > 0.0 && std::isfinite(dmax)) ? 1 : 0;
+        p->fused_enabled = 1;
+        p->step_ok = 0;
+    }
     {
         int dev = 0;
         UPPE_HIP(hipGetDevice(&dev));
@@ -2746,6 +3274,7 @@ int uppe3d_begin(uppe3d_plan *p, double L, double tol, double max_step, double m
     UPPE_CHECK(tol > 0.0 && max_step > 0.0 && min_step >= 0.0,
                "uppe3d_begin: tol and max_step must be positive");
     begin_kernel<<<1, 1, 0, (hipStream_t)stream>>>(p->ctl, L, tol, max_step, min_step);
+    p->step_ok = fused_step_ok(p, max_step);
     phase_kernel<<<(p->d.nt + 255) / 256, 256, 0, (hipStream_t)stream>>>(p->ctl, p->tb, p->ph,
                                                                        p->d.nt);
     UPPE_HIP(hipGetLastError());
@@ -2794,6 +3323,7 @@ int uppe3d_replay_attempt(uppe3d_plan *p, double h, int accept, void *stream) {
     UPPE_CHECK(p != nullptr && h > 0.0, "uppe3d_replay_attempt: bad argument");
     hipStream_t s = (hipStream_t)stream;
     replay_set_kernel<<<1, 1, 0, s>>>(p->ctl, h, accept, h);
+    p->step_ok = fused_step_ok(p, h);
     phase_kernel<<<(p->d.nt + 255) / 256, 256, 0, s>>>(p->ctl, p->tb, p->ph, p->d.nt);
     UPPE_HIP(hipGetLastError());
     return enqueue_attempt(p, s);
@@ -3080,6 +3610,18 @@ int uppe3d_adjoint_step(uppe3d_plan *p, double h, const void *B, const void *X2,
     RET(tadj(p, (const float2 *)X2, a.Yb, a.Xb, nullptr, h, s));
     RET(xadj(p, 2, a, hf, s));
     RET(tadj(p, (const float2 *)B, a.Yb, a.Bb, a.Blin, h, s));
+    return UPPE_OK;
+}
+
+int uppe3d_set_fused(uppe3d_plan *p, int enable) {
+    UPPE_CHECK(p, "uppe3d_set_fused: null plan");
+    p->fused_enabled = enable ? 1 : 0;
+    return UPPE_OK;
+}
+
+int uppe3d_attempt_path(uppe3d_plan *p, int *fused) {
+    UPPE_CHECK(p && fused, "uppe3d_attempt_path: null argument");
+    *fused = use_fused(p) ? 1 : 0;
     return UPPE_OK;
 }
 

@@ -38,6 +38,9 @@ class ERK43IP_UPPE_3D_Optimized:
     valid_mask = True
     # attempts enqueued per library call of propagate/advance (the interrupt granularity)
     attempts_per_call = 64
+    # the restructured attempt (stage fields in (x, y, omega), fused t-passes; 20 field
+    # passes instead of 31) wherever the plan admits it; False forces the 11-launch attempt
+    restructured = True
 
     def __init__(self, x_grid, y_grid, t_grid, medium, precision="single"):
         import torch
@@ -160,6 +163,7 @@ class ERK43IP_UPPE_3D_Optimized:
             ctypes.byref(d), _lib.ptr(self._ws), nbytes, *[_lib.host_ptr(a) for a in arrs],
             _lib.stream_handle(), ctypes.byref(plan)), "uppe3d_plan_create")
         self._plan = plan
+        _lib.check(lib.uppe3d_set_fused(plan, int(bool(self.restructured))), "uppe3d_set_fused")
         if self.error_norm == "l2":
             _lib.check(lib.uppe3d_set_error_norm(plan, _lib.UPPE_NORM_L2),
                        "uppe3d_set_error_norm")
@@ -193,6 +197,13 @@ class ERK43IP_UPPE_3D_Optimized:
         _lib.check(lib.uppe3d_load_field(self._plan, _lib.ptr(a), _lib.stream_handle()),
                    "uppe3d_load_field")
         return a
+
+    def attempt_path(self):
+        """'restructured' or 'classic': the dataflow of the plan's next attempt."""
+        f = ctypes.c_int(0)
+        _lib.check(_lib.load().uppe3d_attempt_path(self._plan, ctypes.byref(f)),
+                   "uppe3d_attempt_path")
+        return "restructured" if f.value else "classic"
 
     def status(self):
         st = _lib.Uppe3DStatus()
@@ -242,11 +253,13 @@ class ERK43IP_UPPE_3D_Optimized:
         self.load(A0)
         s = _lib.stream_handle()
         _lib.check(lib.uppe3d_begin(self._plan, 1e300, 1.0, 1.0, 0.0, s), "uppe3d_begin")
-        errs = []
+        errs, paths = [], []
         for (h, _, acc) in steps:
             _lib.check(lib.uppe3d_replay_attempt(self._plan, float(h), int(bool(acc)), s),
                        "uppe3d_replay_attempt")
             errs.append(self.status().err)
+            paths.append(self.attempt_path())
+        self.replay_paths = paths
         self.last_status = self.status()
         self.replay_errors = errs
         return self._finish(return_gpu)
