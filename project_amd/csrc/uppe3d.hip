@@ -1119,6 +1119,219 @@ __global__ __launch_bounds__(pipe1024::T) void tpass_pipe1024_kernel(
     if (tid < ndef) list[atomicAdd(&ctl->plasma_groups[slot], 1)] = defl[tid];
 }
 
+// ---------------------------------------------------------------------------------------
+// Persistent, software-pipelined fused t-passes for NT = 1024 without plasma (MODE 3 and 4
+// of tfused_group, same arithmetic).  One 512-thread workgroup per CU walks 8-line groups,
+// one line per wavefront.  At the start of each group the rows of its second field (f2) and
+// then those of the next group's first field (f1) are issued into registers (16 per thread
+// each), so both are in flight during the first N evaluation.  As in tpass_pipe1024_kernel,
+// nothing inside the loop reads global memory after those loads are issued except the loads
+// themselves: lane twiddles, the output factor (shock x mask) and the filter sit in LDS, the
+// absorber of the wavefront's line is a scalar load, and deferred groups are listed in LDS.
+// Groups are taken in the xcd_pair_group order of the logical index blockIdx.x + i gridDim.x,
+// so the two halves of a 128-byte row go to workgroups b and b + 8 (one XCD) at the same
+// iteration.  Deferred groups are re-run by tfused_kernel<1024, 8, MODE, true>.
+namespace pipef {
+constexpr int T = 512;                        // threads
+constexpr int L = 8;                          // lines per group
+constexpr int PER = 1024 * L / T;             // rows per thread per field and group (16)
+constexpr int SP = 66;                        // stage pitch per line (float2)
+constexpr int CS = L * SP;                    // stage stride per chunk of 64 rows
+constexpr int TWO = L * W1024::REGION;        // lane twiddles A, B (2 x 1024 float2)
+constexpr int FO = TWO + 2048;                // output factor, 1024 floats
+constexpr int FI = FO + 512;                  // anti-alias filter, 1024 floats
+constexpr int LDS = FI + 512;
+constexpr int MAXDEF = 64;                    // deferred groups per workgroup
+static_assert(16 * CS <= TWO, "the group stage must fit in the transform regions");
+}  // namespace pipef
+
+// N = mask [shock] FFT_t(Kerr) of a line whose inverse transform is in v (time layout of
+// W1024), lane twiddles and the output factor in LDS
+__device__ __forceinline__ void kerr_forward_lds(float2 (&v)[16], float2 *buf, int lane,
+                                                 const float2 *twA, const float2 *twB,
+                                                 const float *fo, const Phys3D &ph) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const float I = ph.a_scale2 * cabs2(v[k]);
+        const float kap = I * (ph.g_n2 + ph.g_n4 * I);
+        v[k] = make_float2(-kap * v[k].y, kap * v[k].x);
+    }
+    w1024_forward_lds(v, buf, lane, twA, twB);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = cscale(v[r], fo[lane + 64 * r]);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(pipef::T) void tfused_pipe_kernel(Fields3D f, Tabs3D tb, Phys3D ph,
+                                                               Ctl3D *ctl, int nx, int ny,
+                                                               int *list, int slot,
+                                                               int ngroups) {
+    using namespace pipef;
+    __shared__ float2 lds[LDS];
+    __shared__ int ndef, defl[MAXDEF];
+    if (ctl->done) return;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    for (int i = tid; i < 1024; i += T) {
+        lds[TWO + i] = tb.twA[i];
+        lds[TWO + 1024 + i] = tb.twB[i];
+    }
+    float *fo = (float *)(lds + FO);
+    for (int i = tid; i < 1024; i += T)
+        fo[i] = ph.use_shock ? tb.shock_w[i] * tb.mask_w[i] : tb.mask_w[i];
+    float *fi = (float *)(lds + FI);
+    if (MODE == 4)
+        for (int i = tid; i < 1024; i += T) fi[i] = tb.filt_w[i];
+    if (tid == 0) ndef = 0;
+    const float h = (float)ctl->h;
+    const float thr = ctl->plasma_thr;
+    float2 *const f1 = f.X, *const f2 = f.Yw;
+    float2 *const bnew = ctl->cur ? f.B0 : f.B1;
+    const int bpx = ny / L;
+    const int plane = nx * ny;
+    const int q = tid % L, row = tid / L;   // row 0..63: rows row + 64 c of the group
+    const int step64 = 64 * plane;
+    const int sw = q * SP + row, sr = wv * SP + lane;
+    const float2 *twA = lds + TWO, *twB = lds + TWO + 1024;
+    float2 *buf = lds + wv * W1024::REGION;
+    float2 g1[PER];
+    int j = blockIdx.x;
+    if (j < ngroups) {
+        const int g = xcd_pair_group(j, ngroups);
+        int off = row * plane + (g / bpx) * ny + (g % bpx) * L + q;
+#pragma unroll
+        for (int c = 0; c < PER; ++c) {
+            g1[c] = f1[off];
+            off = opaque_i(off + step64);
+        }
+    }
+    for (; j < ngroups; j += gridDim.x) {
+        const int grp = xcd_pair_group(j, ngroups);
+        const int col0 = (grp / bpx) * ny + (grp % bpx) * L;
+        const int lo = row * plane + col0 + q;
+        float absorb = 1.0f;
+        if (MODE == 4) absorb = expf(-tb.damp[col0 + wv] * h);   // scalar load
+        __syncthreads();   // the previous group's readers are done with the stage
+#pragma unroll
+        for (int c = 0; c < PER; ++c) lds[sw + c * CS] = g1[c];
+        __syncthreads();
+        float2 a[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) a[c] = lds[sr + c * CS];
+        // in flight during the first N evaluation: this group's f2 rows, then the next
+        // group's f1 rows
+        float2 g2[PER];
+        {
+            int off = lo;
+#pragma unroll
+            for (int c = 0; c < PER; ++c) {
+                g2[c] = f2[off];
+                off = opaque_i(off + step64);
+            }
+        }
+        const int jn = j + gridDim.x;
+        if (jn < ngroups) {
+            const int g = xcd_pair_group(jn, ngroups);
+            int off = row * plane + (g / bpx) * ny + (g % bpx) * L + q;
+#pragma unroll
+            for (int c = 0; c < PER; ++c) {
+                g1[c] = f1[off];
+                off = opaque_i(off + step64);
+            }
+        }
+        __syncthreads();   // stage read: the transform regions alias it
+        float2 v[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) v[c] = a[c];
+        w1024_inverse_lds(v, buf, lane, twA, twB);
+        if (__syncthreads_or(line_imax<16>(v, ph) >= thr ? 1 : 0)) {
+            if (tid == 0) defl[ndef++] = grp;   // the plasma launch runs this group
+            continue;
+        }
+        kerr_forward_lds(v, buf, lane, twA, twB, fo, ph);   // Y2 (MODE 3), Y4 (MODE 4)
+        __syncthreads();   // every transform region is done
+#pragma unroll
+        for (int c = 0; c < PER; ++c) lds[sw + c * CS] = g2[c];
+        __syncthreads();
+        float2 b[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) b[c] = lds[sr + c * CS];
+        __syncthreads();
+        if (MODE == 3) {
+            const float h2 = 0.5f * h, h3 = h / 3.0f;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                a[k] = caxpy(caxpy(a[k], -3.0f, b[k]), h2, v[k]);   // X3
+                b[k] = caxpy(b[k], h3, v[k]);
+            }
+            w1024_inverse_lds(a, buf, lane, twA, twB);
+            if (__syncthreads_or(line_imax<16>(a, ph) >= thr ? 1 : 0)) {
+                if (tid == 0) defl[ndef++] = grp;
+                continue;
+            }
+            kerr_forward_lds(a, buf, lane, twA, twB, fo, ph);   // Y3
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                b[k] = caxpy(b[k], h3, a[k]);
+                a[k] = cscale(a[k], h);
+            }
+        } else {
+            const float h6 = h / 6.0f;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                b[k] = caxpy(b[k], h6, v[k]);   // a4
+                a[k] = cscale(b[k], fi[lane + 64 * k] * absorb);   // B'
+            }
+#pragma unroll
+            for (int c = 0; c < 16; ++c) lds[sr + c * CS] = a[c];
+            __syncthreads();
+            {
+                int off = lo;
+#pragma unroll
+                for (int c = 0; c < PER; ++c) {
+                    bnew[off] = lds[sw + c * CS];
+                    off = opaque_i(off + step64);
+                }
+            }
+            __syncthreads();
+            w1024_inverse_lds(b, buf, lane, twA, twB);
+            if (__syncthreads_or(line_imax<16>(b, ph) >= thr ? 1 : 0)) {
+                if (tid == 0) defl[ndef++] = grp;
+                continue;
+            }
+            kerr_forward_lds(b, buf, lane, twA, twB, fo, ph);   // Y5
+#pragma unroll
+            for (int k = 0; k < 16; ++k) a[k] = csub(v[k], b[k]);   // D = Y4 - Y5
+        }
+        __syncthreads();   // every transform region is done
+#pragma unroll
+        for (int c = 0; c < 16; ++c) lds[sr + c * CS] = a[c];
+        __syncthreads();
+        {
+            int off = lo;
+#pragma unroll
+            for (int c = 0; c < PER; ++c) {
+                f1[off] = lds[sw + c * CS];
+                off = opaque_i(off + step64);
+            }
+        }
+        if (MODE == 3) {
+            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < 16; ++c) lds[sr + c * CS] = b[c];
+            __syncthreads();
+            int off = lo;
+#pragma unroll
+            for (int c = 0; c < PER; ++c) {
+                f2[off] = lds[sw + c * CS];
+                off = opaque_i(off + step64);
+            }
+        }
+    }
+    __syncthreads();
+    if (tid < ndef) list[atomicAdd(&ctl->plasma_groups[slot], 1)] = defl[tid];
+}
+
 // (x, y, t) input -> state B[cur] = FFT_t(A0) / a_scale in the plane-major layout.
 template <int NT, int LINES>
 __global__ __launch_bounds__(64 * LINES) void load_kernel(const float2 *__restrict__ a0,
@@ -2875,11 +3088,29 @@ int launch_tfused(uppe3d_plan *p, int mode, int slot, hipStream_t s) {
     const int groups = nx * (ny / lines);
     const int pg = p->plasma_small ? 8 : p->ncu;
     const dim3 grid(groups), pgrid(groups < pg ? groups : pg);
+    if (p->d.nt == 1024) {
+        // persistent: one workgroup per CU, at most MAXDEF groups each
+        int pgr = p->ncu;
+        if (pgr * pipef::MAXDEF < groups) pgr = (groups + pipef::MAXDEF - 1) / pipef::MAXDEF;
+        if (pgr > groups) pgr = groups;
+        if (mode == 3) {
+            tfused_pipe_kernel<3><<<pgr, pipef::T, 0, s>>>(p->f, p->tb, p->ph, p->ctl, nx, ny,
+                                                           p->list, slot, groups);
+            tfused_kernel<1024, 8, 3, true><<<pgrid, 512, 0, s>>>(p->f, p->tb, p->ph, p->ctl, nx,
+                                                                  ny, p->list, slot);
+        } else {
+            tfused_pipe_kernel<4><<<pgr, pipef::T, 0, s>>>(p->f, p->tb, p->ph, p->ctl, nx, ny,
+                                                           p->list, slot, groups);
+            tfused_kernel<1024, 8, 4, true><<<pgrid, 512, 0, s>>>(p->f, p->tb, p->ph, p->ctl, nx,
+                                                                  ny, p->list, slot);
+        }
+        UPPE_HIP(hipGetLastError());
+        return UPPE_OK;
+    }
     switch (p->d.nt) {
         LAUNCH_F(128, 16)
         LAUNCH_F(256, 16)
         LAUNCH_F(512, 16)
-        LAUNCH_F(1024, 8)
         default:
             return UPPE_EINVAL;
     }
