@@ -2135,15 +2135,21 @@ __global__ __launch_bounds__(x128::T) void xy128f_kernel(Fields3D f, Tabs3D tb, 
     const float2 *B = (ctl->cur ? f.B1 : f.B0) + pb;
     float2 b[16];
     if (MODE == 8) {
-        fft2_128(b, Yp, lds, wv, lane, tb.tw128);
+        float2 E[16];
         fft2_128(a, B, lds, wv, lane, tb.tw128);
 #pragma unroll
         for (int n = 0; n < 16; ++n) {
             const int kx = kxb + (n >> 3) + 16 * (n & 7);
-            const float2 E = eprop_k(tb.kx2[kx] + kyy, w, 0.5f * h, tb, ed, eref);
-            a[n] = cmul(E, caxpy(a[n], 0.5f * h, b[n]));   // X2
-            b[n] = cmul(E, cscale(b[n], h / 6.0f));        // Zh
+            E[n] = eprop_k(tb.kx2[kx] + kyy, w, 0.5f * h, tb, ed, eref);
+            a[n] = cmul(E[n], a[n]);   // E F(B)
             __builtin_amdgcn_sched_barrier(0);   // one element at a time: bounded registers
+        }
+        fft2_128(b, Yp, lds, wv, lane, tb.tw128);
+#pragma unroll
+        for (int n = 0; n < 16; ++n) {
+            const float2 t = cmul(E[n], b[n]);   // E F(Y1)
+            a[n] = caxpy(a[n], 0.5f * h, t);     // X2
+            b[n] = cscale(t, h / 6.0f);          // Zh
         }
         ifft2_128(a, Xp, lds, wv, lane, tb.tw128);
         ifft2_128(b, Yp, lds, wv, lane, tb.tw128);
