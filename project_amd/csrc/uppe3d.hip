@@ -78,6 +78,7 @@ struct Ctl3D {
 struct Tabs3D {
     const float2 *tw_t, *tw_x, *tw_y;
     const float2 *twA, *twB;   // lane twiddles of the NT = 1024 line transform (W1024)
+    const float2 *twC;         // W2048^(lane + 64 r), r < 16: the radix-2 step of W2048
     const float2 *tw128;       // lane twiddles of the 128-point xy transforms (x128)
     const float *d_w, *k_w, *k2_w, *kref_w, *valid_w, *filt_w, *shock_w, *mask_w;
     const float *kx2, *ky2, *damp;
@@ -347,6 +348,48 @@ __device__ __forceinline__ void w1024_forward(float2 (&v)[16], float2 *buf, int 
     Dft<16, 1>::run(v);
 }
 
+// 2048-point line transform (W2048) as two W1024 transforms and one radix-2 step.  Frequency
+// layout as W1024: v[r] = X[lane + 64 r], r < 32 (X[k] and X[k + 1024] in one lane, v[r] and
+// v[r + 16]).  Time layout: v[0..15] hold the even samples x[2 m] and v[16..31] the odd
+// samples x[2 m + 1], each half in the time layout of W1024 (m in place of t).
+//   inverse:  x[2m] = IDFT1024(X[k] + X[k + 1024]) / 2,
+//             x[2m + 1] = IDFT1024((X[k] - X[k + 1024]) W2048^-k) / 2
+//   forward:  X[k] = E[k] + W2048^k O[k],  X[k + 1024] = E[k] - W2048^k O[k]
+//             (E, O the DFT1024 of the even and odd samples)
+// twC[r * 64 + lane] = W2048^(lane + 64 r).  The two halves use the wave's W1024 region in turn.
+__device__ __forceinline__ void w2048_inverse(float2 (&v)[32], float2 *buf, int lane,
+                                              const float2 *__restrict__ twA,
+                                              const float2 *__restrict__ twB,
+                                              const float2 *__restrict__ twC0) {
+    const float2 *twC = opaque(twC0);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const float2 a = v[r], b = v[r + 16];
+        v[r] = cadd(a, b);
+        v[r + 16] = cmulc(csub(a, b), twC[r * 64 + lane]);
+    }
+    w1024_inverse(*reinterpret_cast<float2(*)[16]>(&v[0]), buf, lane, twA, twB);
+    w1024_inverse(*reinterpret_cast<float2(*)[16]>(&v[16]), buf, lane, twA, twB);
+#pragma unroll
+    for (int r = 0; r < 32; ++r) v[r] = cscale(v[r], 0.5f);
+}
+
+__device__ __forceinline__ void w2048_forward(float2 (&v)[32], float2 *buf, int lane,
+                                              const float2 *__restrict__ twA,
+                                              const float2 *__restrict__ twB,
+                                              const float2 *__restrict__ twC0) {
+    w1024_forward(*reinterpret_cast<float2(*)[16]>(&v[0]), buf, lane, twA, twB);
+    w1024_forward(*reinterpret_cast<float2(*)[16]>(&v[16]), buf, lane, twA, twB);
+    const float2 *twC = opaque(twC0);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const float2 t = cmul(v[r + 16], twC[r * 64 + lane]);
+        const float2 e = v[r];
+        v[r] = cadd(e, t);
+        v[r + 16] = csub(e, t);
+    }
+}
+
 // One step of the reference recurrence (media_physics.py:36-63) from rho; returns the
 // plasma coefficients (c1, c2) of the sample and advances rho.  k1_affine: avalanche rate
 // zero, so Gamma = -1/tau_rec and exp(gdt), the step factor are constants.
@@ -537,7 +580,9 @@ __device__ __forceinline__ float line_imax(const float2 (&x)[R], const Phys3D &p
 template <int R, bool FAST>
 __device__ __forceinline__ void line_forward(float2 (&v)[R], float2 *buf, int lane, int rl,
                                              const Tabs3D &tb) {
-    if constexpr (FAST) {
+    if constexpr (FAST && R == 32) {
+        w2048_forward(*reinterpret_cast<float2(*)[32]>(&v[0]), buf, lane, tb.twA, tb.twB, tb.twC);
+    } else if constexpr (FAST) {
         w1024_forward(*reinterpret_cast<float2(*)[16]>(&v[0]), buf, lane, tb.twA, tb.twB);
     } else {
         wave_fft_t<R>(v, lane, rl, tb.tw_t);
@@ -548,14 +593,17 @@ template <int NT, int LINES>
 struct TStage {
     static constexpr int R = NT / 64;
     static constexpr int SP = 66;   // padded stage row (float2): q-stride of 33 dword pairs
-    // lean NT = 1024 instantiation: the line transforms run through LDS (W1024), in a region
-    // shared with the HBM stage (separated by workgroup barriers); that region holds the
-    // whole group, so all rows are loaded (and stored) in one chunk.  Otherwise the stage
-    // holds CH = 4 chunks of 64 rows, double-buffered in registers.
+    // lean NT = 1024 and NT = 2048 instantiations: the line transforms run through LDS (W1024,
+    // W2048), in regions shared with the HBM stage (separated by workgroup barriers).  At
+    // NT = 1024 the stage holds the whole group, so all rows are loaded (and stored) in one
+    // chunk; otherwise the stage holds CH = 4 chunks of 64 rows, double-buffered in registers.
     static constexpr bool fast(bool plasma) {
-        return NT == 1024 && (LINES == 16 || LINES == 8) && !plasma;
+        return ((NT == 1024 && (LINES == 16 || LINES == 8)) || (NT == 2048 && LINES == 8)) &&
+               !plasma;
     }
-    static constexpr int ch(bool plasma) { return fast(plasma) ? R : (R < 4 ? R : 4); }
+    static constexpr int ch(bool plasma) {
+        return fast(plasma) && NT == 1024 ? R : (R < 4 ? R : 4);
+    }
     static constexpr int lds(bool plasma) {
         return fast(plasma) && LINES * W1024::REGION > ch(plasma) * LINES * SP
                    ? LINES * W1024::REGION
@@ -576,7 +624,10 @@ __device__ __forceinline__ bool line_neval(float2 (&v)[NT / 64], float2 *stage, 
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int rl = __brev(lane) >> 26;
     const int sl = FAST ? lane : rl;
-    if constexpr (FAST) {
+    if constexpr (FAST && R == 32) {
+        w2048_inverse(*reinterpret_cast<float2(*)[32]>(&v[0]), stage + wv * W1024::REGION,
+                      lane, tb.twA, tb.twB, tb.twC);
+    } else if constexpr (FAST) {
         w1024_inverse(*reinterpret_cast<float2(*)[16]>(&v[0]), stage + wv * W1024::REGION,
                       lane, tb.twA, tb.twB);
     } else {
@@ -2191,13 +2242,14 @@ __global__ __launch_bounds__(x128::T) void xy128f_kernel(Fields3D f, Tabs3D tb, 
 // The fifth evaluation only feeds the error estimate and has no adjoint.
 
 // LINES t-lines of src (one per wavefront) through the LDS stage: v[r] = src[omega =
-// rl + 64 r] of line lo's column.
-template <int NT, int LINES>
+// sl + 64 r] of line lo's column, sl = lane (W1024 / W2048 layout, FAST) or the bit-reversed
+// lane (wave_fft_t layout).
+template <int NT, int LINES, bool FAST>
 __device__ __forceinline__ void line_load(float2 (&v)[NT / 64], const float2 *__restrict__ src,
                                           float2 *stage, int lo, int plane, int tid) {
     constexpr int R = NT / 64, CH = R < 4 ? R : 4, SP = 66;
     const int q = tid % LINES, row = tid / LINES, wv = tid >> 6;
-    const int rl = __brev(tid & 63) >> 26;
+    const int sl = FAST ? (tid & 63) : (__brev(tid & 63) >> 26);
 #pragma unroll
     for (int c0 = 0; c0 < R; c0 += CH) {
         float2 g[CH];
@@ -2207,23 +2259,23 @@ __device__ __forceinline__ void line_load(float2 (&v)[NT / 64], const float2 *__
         for (int cc = 0; cc < CH; ++cc) stage[(cc * LINES + q) * SP + row] = g[cc];
         __syncthreads();
 #pragma unroll
-        for (int cc = 0; cc < CH; ++cc) v[c0 + cc] = stage[(cc * LINES + wv) * SP + rl];
+        for (int cc = 0; cc < CH; ++cc) v[c0 + cc] = stage[(cc * LINES + wv) * SP + sl];
         __syncthreads();
     }
 }
 
 // inverse of line_load; dst = v (+ acc when acc != nullptr)
-template <int NT, int LINES>
+template <int NT, int LINES, bool FAST>
 __device__ __forceinline__ void line_store(const float2 (&v)[NT / 64], float2 *__restrict__ dst,
                                            const float2 *__restrict__ acc, float2 *stage, int lo,
                                            int plane, int tid) {
     constexpr int R = NT / 64, CH = R < 4 ? R : 4, SP = 66;
     const int q = tid % LINES, row = tid / LINES, wv = tid >> 6;
-    const int rl = __brev(tid & 63) >> 26;
+    const int sl = FAST ? (tid & 63) : (__brev(tid & 63) >> 26);
 #pragma unroll
     for (int c0 = 0; c0 < R; c0 += CH) {
 #pragma unroll
-        for (int cc = 0; cc < CH; ++cc) stage[(cc * LINES + wv) * SP + rl] = v[c0 + cc];
+        for (int cc = 0; cc < CH; ++cc) stage[(cc * LINES + wv) * SP + sl] = v[c0 + cc];
         __syncthreads();
 #pragma unroll
         for (int cc = 0; cc < CH; ++cc) {
@@ -2241,30 +2293,45 @@ __device__ __forceinline__ void line_store(const float2 (&v)[NT / 64], float2 *_
 // gk = -i g (adjoint of the factor i):  Xb = FFT_t((phi + phi' I) gk + phi' s2 x^2 conj(gk)).
 // A line whose plasma term matters at FP32 resolution (peak intensity at or above the step's
 // threshold, plasma_threshold) sets ctl->adj_plasma: the plasma adjoint is not provided for 3D.
+// N_t = 1024 and 2048 (8-line workgroups, XCD-paired) transform through LDS (W1024, W2048) in
+// regions that alias the stage; shorter lines use the wave_fft_t lane transforms.
 template <int NT, int LINES>
 __global__ __launch_bounds__(64 * LINES) void tadj_kernel(const float2 *X, const float2 *Yb,
                                                           float2 *out, const float2 *acc,
                                                           Tabs3D tb, Phys3D ph, Ctl3D *ctl,
                                                           double h, int nx, int ny) {
     constexpr int R = NT / 64, CH = R < 4 ? R : 4, SP = 66;
-    __shared__ float2 stage[CH * LINES * SP];
-    const int tid = threadIdx.x, lane = tid & 63;
+    constexpr bool FAST = (NT == 1024 || NT == 2048) && LINES == 8;
+    constexpr int STG = CH * LINES * SP;
+    __shared__ float2 stage[FAST && LINES * W1024::REGION > STG ? LINES * W1024::REGION : STG];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int rl = __brev(lane) >> 26;
+    const int sl = FAST ? lane : rl;
+    const int blk = LINES == 8 ? xcd_pair_group(blockIdx.x, gridDim.x) : blockIdx.x;
     const int bpx = ny / LINES;
-    const int x0 = blockIdx.x / bpx, y0 = (blockIdx.x % bpx) * LINES;
+    const int x0 = blk / bpx, y0 = (blk % bpx) * LINES;
     const int plane = nx * ny;
     const int lo = (tid / LINES) * plane + x0 * ny + y0 + tid % LINES;
+    float2 *const buf = stage + wv * W1024::REGION;
     float2 x[R], g[R];
-    line_load<NT, LINES>(x, X, stage, lo, plane, tid);
-    line_load<NT, LINES>(g, Yb, stage, lo, plane, tid);
+    line_load<NT, LINES, FAST>(x, X, stage, lo, plane, tid);
+    line_load<NT, LINES, FAST>(g, Yb, stage, lo, plane, tid);
     const bool use_shock = ph.use_shock != 0;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int w = rl + 64 * r;
+        const int w = sl + 64 * r;
         g[r] = cscale(g[r], use_shock ? tb.shock_w[w] * tb.mask_w[w] : tb.mask_w[w]);
     }
-    wave_ifft_t<R>(x, lane, rl, tb.tw_t);
-    wave_ifft_t<R>(g, lane, rl, tb.tw_t);
+    if constexpr (FAST && R == 32) {
+        w2048_inverse(*reinterpret_cast<float2(*)[32]>(&x[0]), buf, lane, tb.twA, tb.twB, tb.twC);
+        w2048_inverse(*reinterpret_cast<float2(*)[32]>(&g[0]), buf, lane, tb.twA, tb.twB, tb.twC);
+    } else if constexpr (FAST) {
+        w1024_inverse(*reinterpret_cast<float2(*)[16]>(&x[0]), buf, lane, tb.twA, tb.twB);
+        w1024_inverse(*reinterpret_cast<float2(*)[16]>(&g[0]), buf, lane, tb.twA, tb.twB);
+    } else {
+        wave_ifft_t<R>(x, lane, rl, tb.tw_t);
+        wave_ifft_t<R>(g, lane, rl, tb.tw_t);
+    }
     if (line_imax<R>(x, ph) >= ctl->plasma_thr && lane == 0) atomicOr(&ctl->adj_plasma, 1);
 #pragma unroll
     for (int k = 0; k < R; ++k) {
@@ -2279,8 +2346,15 @@ __global__ __launch_bounds__(64 * LINES) void tadj_kernel(const float2 *X, const
         x[k] = make_float2(c * gk.x + d * (a2.x * gk.x + a2.y * gk.y),
                            c * gk.y + d * (a2.y * gk.x - a2.x * gk.y));
     }
-    wave_fft_t<R>(x, lane, rl, tb.tw_t);
-    line_store<NT, LINES>(x, out, acc, stage, lo, plane, tid);
+    if constexpr (FAST && R == 32) {
+        w2048_forward(*reinterpret_cast<float2(*)[32]>(&x[0]), buf, lane, tb.twA, tb.twB, tb.twC);
+    } else if constexpr (FAST) {
+        w1024_forward(*reinterpret_cast<float2(*)[16]>(&x[0]), buf, lane, tb.twA, tb.twB);
+    } else {
+        wave_fft_t<R>(x, lane, rl, tb.tw_t);
+    }
+    if constexpr (FAST) __syncthreads();   // every transform region is done (the stage aliases them)
+    line_store<NT, LINES, FAST>(x, out, acc, stage, lo, plane, tid);
 }
 
 struct Adj3D {
@@ -2856,7 +2930,7 @@ constexpr size_t kAlign = 256;
 size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 struct Layout3D {
-    size_t tw_t, tw_x, tw_y, twA, twB, tw128, d64, kref64, ph_d, ph_r, d_w, k_w, k2_w, kref_w, valid_w, filt_w, shock_w, mask_w, kx2, ky2,
+    size_t tw_t, tw_x, tw_y, twA, twB, twC, tw128, d64, kref64, ph_d, ph_r, d_w, k_w, k2_w, kref_w, valid_w, filt_w, shock_w, mask_w, kx2, ky2,
         damp, lut, ctl, list, fields[7], total;
 };
 
@@ -2871,6 +2945,7 @@ Layout3D layout3d(const uppe3d_desc &d) {
     L.tw_t = take(sizeof(float2) * d.nt);
     L.twA = take(sizeof(float2) * 16 * 64);
     L.twB = take(sizeof(float2) * 16 * 64);
+    L.twC = take(sizeof(float2) * 16 * 64);
     L.tw128 = take(sizeof(float2) * 16 * 64);
     L.tw_x = take(sizeof(float2) * d.nx);
     L.tw_y = take(sizeof(float2) * d.ny);
@@ -3319,6 +3394,15 @@ int uppe3d_plan_create(const uppe3d_desc *d, void *workspace, size_t workspace_b
                            hipMemcpyHostToDevice));
         UPPE_HIP(hipMemcpy(ws + L.twB, tbl.data(), sizeof(float2) * tbl.size(),
                            hipMemcpyHostToDevice));
+        // radix-2 step of the NT = 2048 line transform (W2048): W2048^(lane + 64 r)
+        std::vector<float2> tc(16 * 64);
+        for (int r = 0; r < 16; ++r)
+            for (int l = 0; l < 64; ++l) {
+                const double a = -2.0 * M_PI * (double)(l + 64 * r) / 2048.0;
+                tc[r * 64 + l] = make_float2((float)cos(a), (float)sin(a));
+            }
+        UPPE_HIP(hipMemcpy(ws + L.twC, tc.data(), sizeof(float2) * tc.size(),
+                           hipMemcpyHostToDevice));
     }
     UPPE_HIP(hipMemcpy(ws + L.tw_x, twx.data(), sizeof(float2) * nx, hipMemcpyHostToDevice));
     UPPE_HIP(hipMemcpy(ws + L.tw_y, twy.data(), sizeof(float2) * ny, hipMemcpyHostToDevice));
@@ -3374,6 +3458,7 @@ int uppe3d_plan_create(const uppe3d_desc *d, void *workspace, size_t workspace_b
     tb.tw_t = (const float2 *)(ws + L.tw_t);
     tb.twA = (const float2 *)(ws + L.twA);
     tb.twB = (const float2 *)(ws + L.twB);
+    tb.twC = (const float2 *)(ws + L.twC);
     tb.tw128 = (const float2 *)(ws + L.tw128);
     tb.tw_x = (const float2 *)(ws + L.tw_x);
     tb.tw_y = (const float2 *)(ws + L.tw_y);
@@ -3807,7 +3892,7 @@ int tadj(uppe3d_plan *p, const float2 *X, const float2 *Yb, float2 *out, const f
         LAUNCH_TA(128, 16)
         LAUNCH_TA(256, 16)
         LAUNCH_TA(512, 16)
-        LAUNCH_TA(1024, 16)
+        LAUNCH_TA(1024, 8)
         LAUNCH_TA(2048, 8)
         default:
             return UPPE_EINVAL;

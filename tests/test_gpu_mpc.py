@@ -116,7 +116,19 @@ def test_runner_cycle_vs_oracle(gpu, tmp_path):
                                 solvers["fs"].omega, solvers["fs"].m.omega0, s.x, s.y)
     e = rel_l2(out, ref)
     print("cycle rel L2", e, "oracle attempts", [len(lg) for lg in logs])
-    assert e < 2e-4
+    # The cycle's step sequence is set by max_step (20 um in the plates, 5 mm in air: every
+    # attempt's error estimate is below tol and both runs take the same 202 steps), so the
+    # difference is arithmetic: complex64 rounding over 202 steps.  Its scale is that of the
+    # reference's own default arithmetic (precision='single', restated as the oracle in
+    # complex64) on the same cycle.
+    s32 = {k: om.OracleMPC3D(s.x, s.y, s.t, v.m, workers=8, precision="single")
+           for k, v in solvers.items()}
+    ref32, logs32 = om.cavity_cycle(s32, A_in, r.A_scale, seq, r.R_concave, r.T_per_round,
+                                    solvers["fs"].omega, solvers["fs"].m.omega0, s.x, s.y)
+    assert [len(lg) for lg in logs32] == [len(lg) for lg in logs]
+    e32 = rel_l2(ref32, ref)
+    print("FP32 restatement of the reference's cycle: rel L2 %.3e" % e32)
+    assert e < max(1e-5, 1.5 * e32)
     # the cycle's snapshot on the device against the oracle's sums on the oracle field
     g = pm._dg.snapshot(r._held(r._holder[1]), r.w0_waist)
     o = od.snapshot(ref, s.x, s.y, s.t, r.A_scale, r.w0_waist, LAM, s.freq_axis_hz)
