@@ -80,6 +80,7 @@ struct Tabs3D {
     const float2 *twA, *twB;   // lane twiddles of the NT = 1024 line transform (W1024)
     const float2 *twC;         // W2048^(lane + 64 r), r < 16: the radix-2 step of W2048
     const float2 *tw128;       // lane twiddles of the 128-point xy transforms (x128)
+    const float2 *tw256;       // lane twiddles of the 256-point row transforms (r256)
     const float *d_w, *k_w, *k2_w, *kref_w, *valid_w, *filt_w, *shock_w, *mask_w;
     const float *kx2, *ky2, *damp;
     const double *lut;
@@ -1638,8 +1639,7 @@ __device__ __forceinline__ void block_max_to(unsigned int m, unsigned int *dst, 
 // FP64 sum of a per-thread partial over the workgroup, one atomic add per workgroup (the L2
 // error norm).  Called only when ctl->l2 is set, a value uniform over the grid.
 template <int T>
-__device__ __forceinline__ void block_sum_to(float sp, double *dst, double *red, int tid) {
-    double v = (double)sp;
+__device__ __forceinline__ void block_sum_to(double v, double *dst, double *red, int tid) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     if ((tid & 63) == 0) red[tid >> 6] = v;
@@ -2659,57 +2659,111 @@ __global__ __launch_bounds__(1024) void colpass_kernel(const float2 *src, float2
     }
 }
 
-// One row (x fixed) of one plane per wavefront, 16 rows per workgroup.  MODE 2..6: the xy
-// passes of xypass_kernel (inputs already transformed along x); MODE 22..25: the adjoint
-// passes A2..A5 of xadj_kernel.
-template <int NY, int MODE>
-__global__ __launch_bounds__(1024) void row_kernel(Fields3D f, Adj3D a, Tabs3D tb, Ctl3D *ctl,
-                                                   float h_adj, int nx) {
-    constexpr int EPT = NY / 64, LP = NY + NY / 16;
+// Row kernel: one row (x fixed) of one plane per 16 lanes, 16 rows per workgroup.  MODE 2..6:
+// the xy passes of xypass_kernel (inputs already transformed along x); MODE 22..25: the
+// adjoint passes A2..A5 of xadj_kernel.
+// 256-point row transforms held in registers (r256): 16 lanes per row, 16 values per lane,
+// four rows per wavefront.  n = j + 16 r (j = lane & 15, r < 16), k = k1 + 16 k2:
+//   forward: DFT16 over r -> x W256^(j k1) -> [wave-private LDS transpose within the 16 lanes
+//   of the row] -> DFT16 over j; lane (s = lane >> 4, k1 = lane & 15) then holds
+//   v[k2] = X[k1 + 16 k2] of row s.  The inverse is the transposed sequence with conjugate
+//   twiddles (unnormalised).  The input and output of a lane are y = j + 16 r (natural
+//   order), so a row's loads and stores are 128-byte coalesced for every r.
+// tw256[k * 64 + lane] = W256^((lane & 15) k).
+namespace r256 {
+constexpr int PK = 65;             // transpose pitch per k (float2)
+constexpr int WREG = 16 * PK;      // float2 per wavefront
+}  // namespace r256
+
+template <int DIR>
+__device__ __forceinline__ void twiddle256(float2 (&v)[16], int lane,
+                                           const float2 *__restrict__ tw0) {
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+        const float2 *tw = opaque(tw0);
+#pragma unroll
+        for (int k = (g == 0 ? 1 : 8); k < (g == 0 ? 8 : 16); ++k)
+            v[k] = DIR > 0 ? cmul(v[k], tw[k * 64 + lane]) : cmulc(v[k], tw[k * 64 + lane]);
+    }
+}
+
+__device__ __forceinline__ void fwd256(float2 (&v)[16], float2 *wb, int lane,
+                                       const float2 *__restrict__ tw) {
+    Dft<16, 1>::run(v);
+    twiddle256<1>(v, lane, tw);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) wb[k * r256::PK + lane] = v[k];
+    wave_lds_sync();
+    const int s16 = lane & 48, k1 = lane & 15;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) v[j] = wb[k1 * r256::PK + s16 + j];
+    wave_lds_sync();
+    Dft<16, 1>::run(v);
+}
+
+__device__ __forceinline__ void inv256(float2 (&v)[16], float2 *wb, int lane,
+                                       const float2 *__restrict__ tw) {
+    Dft<16, -1>::run(v);   // over k2: lane (s, k1) holds u_k1[j], j < 16
+    const int s16 = lane & 48, k1 = lane & 15;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) wb[j * r256::PK + s16 + k1] = v[j];
+    wave_lds_sync();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = wb[k1 * r256::PK + s16 + k];   // lane (s, j = k1)
+    wave_lds_sync();
+    twiddle256<-1>(v, lane, tw);
+    Dft<16, -1>::run(v);
+}
+
+// The row kernel with the r256 transforms: 16 rows per 256-thread workgroup, four per
+// wavefront; same modes and arithmetic.
+// a row's values y = j + 16 r (src points at the lane's first) -> forward r256 transform
+__device__ __forceinline__ void row256_fwd(float2 (&v)[16], const float2 *__restrict__ src,
+                                           float2 *wb, int lane, const float2 *tw) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = src[16 * r];
+    fwd256(v, wb, lane, tw);
+}
+
+// inverse r256 transform -> dst[16 r] = scale v[r] (dst points at the lane's first value)
+__device__ __forceinline__ void row256_inv(float2 (&v)[16], float2 *__restrict__ dst,
+                                           float scale, float2 *wb, int lane,
+                                           const float2 *tw) {
+    inv256(v, wb, lane, tw);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) dst[16 * r] = cscale(v[r], scale);
+}
+
+// One group of 16 rows (blk: plane blk / (nx / 16), rows 16 (blk % (nx / 16)) + 0..15);
+// the error-norm modes add their squares to m (max) and sq.
+template <int MODE>
+__device__ __forceinline__ void row256_group(int blk, Fields3D &f, Adj3D &a, const Tabs3D &tb,
+                                             const Ctl3D *ctl, float h_adj, int nx,
+                                             float2 *wregs, unsigned int &m, double &sq) {
+    constexpr int NY = 256;
     constexpr bool ADJ = MODE >= 20;
-    __shared__ float2 lines[16 * LP];
-    __shared__ unsigned int red[16];
-    __shared__ double reds[16];
-    if (!ADJ && ctl->done) return;
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int rpp = nx / 16;
-    const int w = blockIdx.x / rpp, x = (blockIdx.x % rpp) * 16 + wv;
+    const int w = blk / rpp, x = (blk % rpp) * 16 + 4 * wv + (lane >> 4);
+    const int j = lane & 15;
     const size_t rb = ((size_t)w * nx + x) * NY;
-    float2 *line = lines + wv * LP;
+    float2 *wb = wregs + wv * r256::WREG;
     const float h = ADJ ? h_adj : (float)ctl->h;
     const float2 ed = tb.ph_d[w], eref = tb.ph_r[w];
     const float kxx = tb.kx2[x];
     const float inv = 1.0f / ((float)nx * (float)NY);
-    // forward transform of a row of src into v (v[r] = k_y = lane + 64 r)
-    auto fwd = [&](const float2 *src, float2 (&v)[EPT]) {
-#pragma unroll
-        for (int r = 0; r < EPT; ++r) line[pidx(lane + 64 * r)] = src[rb + lane + 64 * r];
-        wave_lds_sync();
-        LineFFT<NY, EPT, 1, 1>::run(line, lane, tb.tw_y);
-#pragma unroll
-        for (int r = 0; r < EPT; ++r) v[r] = line[pidx(lane + 64 * r)];
-        wave_lds_sync();
-    };
-    auto inv_store = [&](const float2 (&v)[EPT], float2 *dst, float scale) {
-#pragma unroll
-        for (int r = 0; r < EPT; ++r) line[pidx(lane + 64 * r)] = v[r];
-        wave_lds_sync();
-        LineFFT<NY, EPT, 1, -1>::run(line, lane, tb.tw_y);
-#pragma unroll
-        for (int r = 0; r < EPT; ++r) dst[rb + lane + 64 * r] = cscale(line[pidx(lane + 64 * r)], scale);
-        wave_lds_sync();
-    };
+    // value r of a lane: y = j + 16 r before the forward transform and after the inverse,
+    // k_y = j + 16 r in between
+    const float2 *const tw = tb.tw256;
     auto E_at = [&](int r) {
-        return eprop_k(kxx + tb.ky2[lane + 64 * r], w, 0.5f * h, tb, ed, eref);
+        return eprop_k(kxx + tb.ky2[j + 16 * r], w, 0.5f * h, tb, ed, eref);
     };
-    float2 b[EPT], o[EPT];
+    float2 b[16], o[16];
     const float2 *in = (MODE == 5) ? f.X : (ADJ ? a.Xb : f.Yw);
-    fwd(in, b);
-    unsigned int m = 0;
-    float sq = 0.0f;
+    row256_fwd(b, in + rb + j, wb, lane, tw);
 #pragma unroll
-    for (int r = 0; r < EPT; ++r) {
-        const size_t e = rb + lane + 64 * r;
+    for (int r = 0; r < 16; ++r) {
+        const size_t e = rb + j + 16 * r;
         if (MODE == 3) {
             o[r] = caxpy(f.V[e], 0.5f * h, b[r]);
             f.Q[e] = caxpy(f.Q[e], h / 3.0f, b[r]);
@@ -2722,12 +2776,12 @@ __global__ __launch_bounds__(1024) void row_kernel(Fields3D f, Adj3D a, Tabs3D t
             const float a2 = cabs2(o[r]);
             const unsigned int u = __float_as_uint(a2);
             m = u > m ? u : m;
-            sq += a2;
+            sq += (double)a2;
         } else if (MODE == 6) {
             const float a2 = cabs2(b[r]);
             const unsigned int u = __float_as_uint(a2);
             m = u > m ? u : m;
-            sq += a2;
+            sq += (double)a2;
         } else if (MODE == 25) {
             a.Qb[e] = cscale(b[r], inv);
         } else if (MODE == 24) {
@@ -2742,11 +2796,11 @@ __global__ __launch_bounds__(1024) void row_kernel(Fields3D f, Adj3D a, Tabs3D t
         }
     }
     if (MODE == 2) {
-        float2 av[EPT];
-        fwd(f.X, av);   // state, transformed along x
+        float2 av[16];
+        row256_fwd(av, f.X + rb + j, wb, lane, tw);   // state, transformed along x
 #pragma unroll
-        for (int r = 0; r < EPT; ++r) {
-            const size_t e = rb + lane + 64 * r;
+        for (int r = 0; r < 16; ++r) {
+            const size_t e = rb + j + 16 * r;
             const float2 E = E_at(r);
             f.V[e] = cmul(E, av[r]);
             f.Q[e] = cmul(E, caxpy(av[r], h / 6.0f, b[r]));
@@ -2754,22 +2808,41 @@ __global__ __launch_bounds__(1024) void row_kernel(Fields3D f, Adj3D a, Tabs3D t
         }
     }
     if (MODE == 22) {
-        float2 ab[EPT];
+        float2 ab[16];
 #pragma unroll
-        for (int r = 0; r < EPT; ++r) {
-            const size_t e = rb + lane + 64 * r;
+        for (int r = 0; r < 16; ++r) {
+            const size_t e = rb + j + 16 * r;
             const float2 U = cscale(b[r], inv), q = a.Qb[e], E = E_at(r);
             ab[r] = cmulc(cadd(cadd(U, a.Vb[e]), q), E);
             o[r] = cmulc(caxpy(cscale(U, 0.5f * h), h / 6.0f, q), E);
         }
-        inv_store(ab, a.Blin, 1.0f);
+        row256_inv(ab, a.Blin + rb + j, 1.0f, wb, lane, tw);
     }
+    if (MODE == 2 || MODE == 3 || MODE == 4 || MODE == 5)
+        row256_inv(o, f.X + rb + j, 1.0f / (float)NY, wb, lane, tw);
+    if (MODE == 22 || MODE == 23 || MODE == 24) row256_inv(o, a.Yb + rb + j, 1.0f, wb, lane, tw);
+}
+
+// The row kernel: 16 rows per 256-thread workgroup, four per wavefront.  The error-norm modes
+// (5, 6) loop over several row groups per workgroup (grid-stride) so that one atomic per
+// workgroup, not per row group, reaches the control block.
+template <int MODE>
+__global__ __launch_bounds__(256) void row256_kernel(Fields3D f, Adj3D a, Tabs3D tb, Ctl3D *ctl,
+                                                     float h_adj, int nx, int ngroups) {
+    constexpr bool ADJ = MODE >= 20;
+    __shared__ float2 wregs[4 * r256::WREG];
+    __shared__ unsigned int red[4];
+    __shared__ double reds[4];
+    if (!ADJ && ctl->done) return;
+    unsigned int m = 0;
+    double sq = 0.0;
+    for (int blk = blockIdx.x; blk < ngroups; blk += gridDim.x)
+        row256_group<MODE>(blk, f, a, tb, ctl, h_adj, nx, wregs, m, sq);
     if (MODE == 5 || MODE == 6) {
-        block_max_to<1024>(m, MODE == 5 ? &ctl->maxA_bits : &ctl->maxD_bits, red, tid);
-        if (ctl->l2) block_sum_to<1024>(sq, MODE == 5 ? &ctl->sumA : &ctl->sumD, reds, tid);
+        const int tid = threadIdx.x;
+        block_max_to<256>(m, MODE == 5 ? &ctl->maxA_bits : &ctl->maxD_bits, red, tid);
+        if (ctl->l2) block_sum_to<256>(sq, MODE == 5 ? &ctl->sumA : &ctl->sumD, reds, tid);
     }
-    if (MODE == 2 || MODE == 3 || MODE == 4 || MODE == 5) inv_store(o, f.X, 1.0f / (float)NY);
-    if (MODE == 22 || MODE == 23 || MODE == 24) inv_store(o, a.Yb, 1.0f);
 }
 
 // adjoint A5 prelude: Xb = filter absorber Bb', Yb4 = h/6 Xb
@@ -2930,7 +3003,7 @@ constexpr size_t kAlign = 256;
 size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 struct Layout3D {
-    size_t tw_t, tw_x, tw_y, twA, twB, twC, tw128, d64, kref64, ph_d, ph_r, d_w, k_w, k2_w, kref_w, valid_w, filt_w, shock_w, mask_w, kx2, ky2,
+    size_t tw_t, tw_x, tw_y, twA, twB, twC, tw128, tw256, d64, kref64, ph_d, ph_r, d_w, k_w, k2_w, kref_w, valid_w, filt_w, shock_w, mask_w, kx2, ky2,
         damp, lut, ctl, list, fields[7], total;
 };
 
@@ -2947,6 +3020,7 @@ Layout3D layout3d(const uppe3d_desc &d) {
     L.twB = take(sizeof(float2) * 16 * 64);
     L.twC = take(sizeof(float2) * 16 * 64);
     L.tw128 = take(sizeof(float2) * 16 * 64);
+    L.tw256 = take(sizeof(float2) * 16 * 64);
     L.tw_x = take(sizeof(float2) * d.nx);
     L.tw_y = take(sizeof(float2) * d.ny);
     L.d64 = take(8 * d.nt);
@@ -3075,6 +3149,8 @@ int launch_xy(uppe3d_plan *p, int mode, hipStream_t s) {
             // column pass -> row kernel -> inverse column pass (planes beyond LDS)
             const int nt = p->d.nt, ny = p->d.ny;
             const dim3 gc(nt * (ny / 16)), gr(nt * (p->d.nx / 16)), blk(1024);
+            // error-norm row passes: a grid-stride loop, one atomic per workgroup
+            const dim3 gred(gr.x < 8u * (unsigned)p->ncu ? gr.x : 8u * (unsigned)p->ncu);
             Fields3D &f = p->f;
             const Adj3D none = {};
             const float inx = 1.0f / (float)p->d.nx;
@@ -3090,27 +3166,27 @@ int launch_xy(uppe3d_plan *p, int mode, hipStream_t s) {
                 case 2:
                     colF(nullptr, f.X, 1);
                     colF(f.Yw, f.Yw, 0);
-                    row_kernel<256, 2><<<gr, blk, 0, s>>>(f, none, p->tb, p->ctl, 0.0f, p->d.nx);
+                    row256_kernel<2><<<gr, 256, 0, s>>>(f, none, p->tb, p->ctl, 0.0f, p->d.nx, (int)gr.x);
                     colI(f.X);
                     break;
                 case 3:
                     colF(f.Yw, f.Yw, 0);
-                    row_kernel<256, 3><<<gr, blk, 0, s>>>(f, none, p->tb, p->ctl, 0.0f, p->d.nx);
+                    row256_kernel<3><<<gr, 256, 0, s>>>(f, none, p->tb, p->ctl, 0.0f, p->d.nx, (int)gr.x);
                     colI(f.X);
                     break;
                 case 4:
                     colF(f.Yw, f.Yw, 0);
-                    row_kernel<256, 4><<<gr, blk, 0, s>>>(f, none, p->tb, p->ctl, 0.0f, p->d.nx);
+                    row256_kernel<4><<<gr, 256, 0, s>>>(f, none, p->tb, p->ctl, 0.0f, p->d.nx, (int)gr.x);
                     colI(f.X);
                     break;
                 case 5:
                     colF(f.Y4, f.X, 0);
-                    row_kernel<256, 5><<<gr, blk, 0, s>>>(f, none, p->tb, p->ctl, 0.0f, p->d.nx);
+                    row256_kernel<5><<<gred, 256, 0, s>>>(f, none, p->tb, p->ctl, 0.0f, p->d.nx, (int)gr.x);
                     colI(f.X);
                     break;
                 default:
                     colF(f.Yw, f.Yw, 0);
-                    row_kernel<256, 6><<<gr, blk, 0, s>>>(f, none, p->tb, p->ctl, 0.0f, p->d.nx);
+                    row256_kernel<6><<<gred, 256, 0, s>>>(f, none, p->tb, p->ctl, 0.0f, p->d.nx, (int)gr.x);
                     break;
             }
             UPPE_HIP(hipGetLastError());
@@ -3394,6 +3470,15 @@ int uppe3d_plan_create(const uppe3d_desc *d, void *workspace, size_t workspace_b
                            hipMemcpyHostToDevice));
         UPPE_HIP(hipMemcpy(ws + L.twB, tbl.data(), sizeof(float2) * tbl.size(),
                            hipMemcpyHostToDevice));
+        // 256-point row transforms (r256): W256^((lane & 15) k)
+        std::vector<float2> t256(16 * 64);
+        for (int k = 0; k < 16; ++k)
+            for (int l = 0; l < 64; ++l) {
+                const double a = -2.0 * M_PI * (double)((l & 15) * k) / 256.0;
+                t256[k * 64 + l] = make_float2((float)cos(a), (float)sin(a));
+            }
+        UPPE_HIP(hipMemcpy(ws + L.tw256, t256.data(), sizeof(float2) * t256.size(),
+                           hipMemcpyHostToDevice));
         // radix-2 step of the NT = 2048 line transform (W2048): W2048^(lane + 64 r)
         std::vector<float2> tc(16 * 64);
         for (int r = 0; r < 16; ++r)
@@ -3460,6 +3545,7 @@ int uppe3d_plan_create(const uppe3d_desc *d, void *workspace, size_t workspace_b
     tb.twB = (const float2 *)(ws + L.twB);
     tb.twC = (const float2 *)(ws + L.twC);
     tb.tw128 = (const float2 *)(ws + L.tw128);
+    tb.tw256 = (const float2 *)(ws + L.tw256);
     tb.tw_x = (const float2 *)(ws + L.tw_x);
     tb.tw_y = (const float2 *)(ws + L.tw_y);
     tb.d_w = (const float *)(ws + L.d_w);
@@ -3844,21 +3930,21 @@ int xadj256(uppe3d_plan *p, int mode, const Adj3D &a, float h, hipStream_t s) {
         case 5:
             a5pre_kernel<<<2048, 256, 0, s>>>(a, p->tb, h, nx * ny, (size_t)nx * ny * nt);
             col(a.Xb, 1);
-            row_kernel<256, 25><<<gr, blk, 0, s>>>(p->f, a, p->tb, p->ctl, h, nx);
+            row256_kernel<25><<<gr, 256, 0, s>>>(p->f, a, p->tb, p->ctl, h, nx, (int)gr.x);
             break;
         case 4:
             col(a.Xb, 1);
-            row_kernel<256, 24><<<gr, blk, 0, s>>>(p->f, a, p->tb, p->ctl, h, nx);
+            row256_kernel<24><<<gr, 256, 0, s>>>(p->f, a, p->tb, p->ctl, h, nx, (int)gr.x);
             col(a.Yb, -1);
             break;
         case 3:
             col(a.Xb, 1);
-            row_kernel<256, 23><<<gr, blk, 0, s>>>(p->f, a, p->tb, p->ctl, h, nx);
+            row256_kernel<23><<<gr, 256, 0, s>>>(p->f, a, p->tb, p->ctl, h, nx, (int)gr.x);
             col(a.Yb, -1);
             break;
         default:
             col(a.Xb, 1);
-            row_kernel<256, 22><<<gr, blk, 0, s>>>(p->f, a, p->tb, p->ctl, h, nx);
+            row256_kernel<22><<<gr, 256, 0, s>>>(p->f, a, p->tb, p->ctl, h, nx, (int)gr.x);
             col(a.Yb, -1);
             col(a.Blin, -1);
             break;

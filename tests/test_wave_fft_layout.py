@@ -130,6 +130,66 @@ def test_w1024_index_algebra():
                                atol=1e-10)
 
 
+def test_w2048_two_w1024_and_radix2():
+    """W2048 (uppe3d.hip): v[lane, r] = X[lane + 64 r], r < 32.  Inverse: e = X[k] + X[k +
+    1024], o = (X[k] - X[k + 1024]) conj(W2048^k), two W1024 inverses, / 2: v[:, :16] hold the
+    even samples and v[:, 16:] the odd ones, each in the W1024 time layout.  Forward: the
+    transposed sequence, X[k] = E + W2048^k O, X[k + 1024] = E - W2048^k O."""
+    twA, twB = w1024_tables()
+    lane = np.arange(64)[:, None]
+    r = np.arange(16)[None, :]
+    twC = np.exp(-2j * np.pi * (lane + 64 * r) / 2048)     # twC[r * 64 + lane] as [lane, r]
+    rng = np.random.default_rng(13)
+    X = rng.standard_normal(2048) + 1j * rng.standard_normal(2048)
+    v = X[np.arange(64)[:, None] + 64 * np.arange(32)[None, :]]
+    a, b = v[:, :16], v[:, 16:]
+    e = w1024_inverse(a + b, twA, twB) / 2
+    o = w1024_inverse((a - b) * np.conj(twC), twA, twB) / 2
+    tix = w1024_time_index()
+    x = np.fft.ifft(X)
+    np.testing.assert_allclose(e, x[2 * tix], atol=1e-14)
+    np.testing.assert_allclose(o, x[2 * tix + 1], atol=1e-14)
+    y = rng.standard_normal(2048) + 1j * rng.standard_normal(2048)
+    E = w1024_forward(y[2 * tix], twA, twB)
+    O = w1024_forward(y[2 * tix + 1], twA, twB) * twC
+    V = np.concatenate([E + O, E - O], axis=1)
+    np.testing.assert_allclose(V, np.fft.fft(y)[np.arange(64)[:, None] + 64 * np.arange(32)[None, :]],
+                               atol=1e-10)
+
+
+def test_r256_row_algebra():
+    """r256 (uppe3d.hip): four 256-point rows per wavefront, lane = 16 s + j, value r of row s
+    at y = j + 16 r.  Forward: DFT16 over r, x W256^(j k1), write wb[k1 * 65 + lane], read
+    wb[k1 * 65 + 16 s + j'] (lane 16 s + k1), DFT16 over j' -> v[k2] = X[k1 + 16 k2].  Inverse:
+    DFT16^-1 over k2, write wb[j * 65 + 16 s + k1], read wb[k1 * 65 + 16 s + k] (lane 16 s + j),
+    x conj(W256^(j k)), DFT16^-1 -> 256 x[j + 16 r]."""
+    PK256, WR = 65, 16 * 65
+    lane = np.arange(64)
+    s16, j = lane & 48, lane & 15
+    tw = np.exp(-2j * np.pi * (j[:, None] * np.arange(16)[None, :]) / 256)   # [lane, k]
+    rng = np.random.default_rng(17)
+    rows = rng.standard_normal((4, 256)) + 1j * rng.standard_normal((4, 256))
+    v = rows[(lane >> 4)[:, None], j[:, None] + 16 * np.arange(16)[None, :]]
+    v = np.fft.fft(v, axis=1) * tw
+    wb = np.full(WR, np.nan + 0j)
+    addr = np.arange(16)[None, :] * PK256 + lane[:, None]
+    assert len(np.unique(addr)) == addr.size and addr.max() < WR
+    wb[addr] = v
+    v = np.fft.fft(wb[j[:, None] * PK256 + s16[:, None] + np.arange(16)[None, :]], axis=1)
+    K = np.fft.fft(rows, axis=1)
+    np.testing.assert_allclose(v, K[(lane >> 4)[:, None], j[:, None] + 16 * np.arange(16)[None, :]],
+                               atol=1e-10)
+    u = np.fft.ifft(v, axis=1) * 16
+    wb = np.full(WR, np.nan + 0j)
+    addr = np.arange(16)[None, :] * PK256 + s16[:, None] + j[:, None]
+    assert len(np.unique(addr)) == addr.size and addr.max() < WR
+    wb[addr] = u
+    w = wb[j[:, None] * PK256 + s16[:, None] + np.arange(16)[None, :]]
+    w = np.fft.ifft(w * np.conj(tw), axis=1) * 16
+    np.testing.assert_allclose(w / 256, rows[(lane >> 4)[:, None], j[:, None] + 16 * np.arange(16)[None, :]],
+                               atol=1e-12)
+
+
 # -- 128 x 128 plane transform: 16 x 8 register DFTs per 8 lanes, wave and plane transposes --
 PK, WREG, PP = 65, 16 * 65, 129
 
