@@ -269,30 +269,40 @@ def bench_cavity(passes=2, slices=20, cpu=True):
     return out
 
 
-def bench_mpc(steps=60):
+def bench_mpc(steps=60, nts=(2046, 2048)):
     """The MPC runner's fused-silica segment (src/3+1D-ERK4(3)IP UPPE/sim.py) at the runner's
-    grid, 256 x 256 x 2048 over 2 mm and 80 ps (N_t 2048 for the runner's 2046), the runner's
-    100 uJ 10.7 ps eigenmode pulse at 1064 nm: `steps` steps of the 20 um maximum step (L2 error
-    norm, tol 1e-5) on the device, and the time of one full cavity cycle projected from it."""
+    grid, 256 x 256 x N_t over 2 mm and 80 ps, the runner's 100 uJ 10.7 ps eigenmode pulse at
+    1064 nm: `steps` steps of the 20 um maximum step (L2 error norm, tol 1e-5) on the device,
+    and the time of one full cavity cycle projected from it.  N_t 2046 is the runner's own t
+    axis (mixed-radix line transforms); 2048 the power-of-two one beside it."""
     import torch
     from project_amd import mpc
-    r = mpc.MPC_Final_Runner(output_dir="", verbose=False)
-    A = r.generate_initial_field()
-    L = steps * r.d_fs / 600.0    # 600 steps of 20 um per 12 mm plate
-    r.propagate_segment(A, 4 * 20e-6, "fs")   # warm-up (plan, tables, code objects)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    r.propagate_segment(A, L, "fs")
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    st = r.solver_fs.last_status
-    per = wall / max(st.attempts, 1)
-    plates = 2 * r.n_plates * r.d_fs / 20e-6
-    return {"workload": "MPC runner fused-silica segment (sim.py), 256x256x2048, 1064 nm, "
-                        "100 uJ 10.7 ps, max_step 20 um, L2 norm, tol 1e-5",
-            "attempts": int(st.attempts), "accepted": int(st.accepted), "wall_s": round(wall, 4),
-            "ms_per_attempt": round(1e3 * per, 3),
-            "projected_cycle_s_plates": round(plates * per, 1)}
+    out = {}
+    for nt in nts:
+        r = mpc.MPC_Final_Runner(output_dir="", verbose=False, Nt=nt)
+        A = r.generate_initial_field()
+        L = steps * r.d_fs / 600.0    # 600 steps of 20 um per 12 mm plate
+        r.propagate_segment(A, 4 * 20e-6, "fs")   # warm-up (plan, tables, code objects)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r.propagate_segment(A, L, "fs")
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        st = r.solver_fs.last_status
+        per = wall / max(st.attempts, 1)
+        plates = 2 * r.n_plates * r.d_fs / 20e-6
+        res = {"workload": "MPC runner fused-silica segment (sim.py), 256x256x%d, 1064 nm, "
+                           "100 uJ 10.7 ps, max_step 20 um, L2 norm, tol 1e-5" % nt,
+               "attempts": int(st.attempts), "accepted": int(st.accepted),
+               "wall_s": round(wall, 4), "ms_per_attempt": round(1e3 * per, 3),
+               "projected_cycle_s_plates": round(plates * per, 1)}
+        if not out:
+            out = res
+        else:
+            out["nt_%d" % nt] = res
+        del r, A
+        torch.cuda.empty_cache()
+    return out
 
 
 def bench_realfield(nx=128, nt=1024, attempts=10, cpu=True):

@@ -38,7 +38,10 @@ enum {
 typedef struct uppe3d_plan uppe3d_plan;
 
 typedef struct {
-    int nx, ny, nt;        /* grid; nx = ny in {16,...,256}, nt in {128,...,2048}, powers of 2 */
+    int nx, ny, nt;        /* grid; nx = ny in {16,...,256} (powers of 2); 128 <= nt <= 2048
+                              with prime factors <= 31, at most one of them above 13 (powers of
+                              2 on register transforms, others such as the MPC runner's 2046 on
+                              mixed-radix LDS transforms); the adjoint takes powers of 2 only */
     double dt;             /* t[1] - t[0] (s) */
     double omega0;         /* carrier angular frequency (rad/s) */
     double a_scale;        /* max|A0| (solver_3d.py:166-168); the state is A / a_scale */

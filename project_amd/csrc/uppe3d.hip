@@ -34,8 +34,10 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "abi_common.hpp"
@@ -2971,6 +2973,534 @@ __global__ void replay_set_kernel(Ctl3D *c, double h, int acc, double h_next) {
     c->done = 0;
 }
 
+// ---------------------------------------------------------------------------------------
+// t axes of any length 128 <= N_t <= 2048 whose prime factors are at most 31, such as the MPC
+// runner's N_t = 2046 = 2 * 3 * 11 * 31 (src/3+1D-ERK4(3)IP UPPE/sim.py:48), where cuFFT /
+// pocketfft take any length.  The power-of-two kernels above hold a line in registers in a
+// layout tied to N_t = 64 R; here a line sits in a wave-private LDS region in natural order and
+// is transformed by mixed-radix Stockham stages (radices ascending, the largest last).  Each
+// stage but the last reads all of its inputs into registers before it writes, so one region
+// per line suffices; the last stage writes exactly the positions it read (Ns R = N), so it runs
+// butterfly by butterfly.  Radices 2, 4, 8, 16 use the radix-2 register DFTs; odd primes use
+// the symmetric direct DFT below (pair sums and differences, (R - 1)^2 / 2 real products per
+// output pair, roots of unity as compile-time constants).  Time-domain work takes a contiguous
+// chunk of C = ceil(N_t / 64) samples per lane, so time order is lane order, as the plasma
+// recurrence needs.
+namespace gl {
+constexpr int NTMAX = 2048;
+constexpr int LINES = 8;                    // lines (wavefronts) per t-pass workgroup
+constexpr int T = 64 * LINES;
+constexpr int CM = NTMAX / 64;              // samples per lane, at most
+constexpr int LP = NTMAX + NTMAX / 16;      // line pitch (float2): pidx(NTMAX - 1) < LP
+constexpr int MAXST = 12;                   // stages (2^11 needs 11 radix-2 stages at most)
+constexpr double PI = 3.141592653589793238462643383279502884;
+
+// cos and sin of 2 pi q / R by Taylor series, evaluated by the compiler only
+constexpr double tcos(double x) {
+    double t = 1.0, s = 1.0;
+    for (int n = 1; n < 40; ++n) {
+        t *= -x * x / ((2.0 * n - 1.0) * (2.0 * n));
+        s += t;
+    }
+    return s;
+}
+constexpr double tsin(double x) {
+    double t = x, s = x;
+    for (int n = 1; n < 40; ++n) {
+        t *= -x * x / ((2.0 * n) * (2.0 * n + 1.0));
+        s += t;
+    }
+    return s;
+}
+template <int R, int Q>
+struct Root {
+    static constexpr double a = 2.0 * PI * (double)(2 * Q <= R ? Q : Q - R) / (double)R;
+    static constexpr float c = (float)tcos(a);
+    static constexpr float s = (float)tsin(a);
+};
+
+template <int B, int E, class F>
+__device__ __forceinline__ void sfor(F &&f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        sfor<B + 1, E>(f);
+    }
+}
+}  // namespace gl
+
+// Length-R DFT (R odd) in registers, natural order in and out; DIR = +1 forward
+// (exp(-2 pi i m k / R)), -1 inverse, unnormalised.  With s_m = v_m + v_{R-m},
+// d_m = v_m - v_{R-m} (m = 1..(R-1)/2) and theta = 2 pi m k / R:
+//   X_k = v_0 + sum s_m cos(theta) - DIR i sum d_m sin(theta),  X_{R-k}: the sign of i flipped.
+template <int R, int DIR>
+__device__ __forceinline__ void dft_odd(float2 (&v)[R]) {
+    constexpr int H = (R - 1) / 2;
+    gl::sfor<1, H + 1>([&](auto mi) {
+        constexpr int m = decltype(mi)::value;
+        const float2 a = v[m], b = v[R - m];
+        v[m] = cadd(a, b);
+        v[R - m] = csub(a, b);
+    });
+    float2 o[R];
+    float2 s0 = v[0];
+    gl::sfor<1, H + 1>([&](auto mi) { s0 = cadd(s0, v[decltype(mi)::value]); });
+    o[0] = s0;
+    gl::sfor<1, H + 1>([&](auto ki) {
+        constexpr int k = decltype(ki)::value;
+        float2 A = v[0], Bs = make_float2(0.0f, 0.0f);
+        gl::sfor<1, H + 1>([&](auto mi) {
+            constexpr int m = decltype(mi)::value;
+            constexpr int q = (m * k) % R;
+            A = caxpy(A, gl::Root<R, q>::c, v[m]);
+            Bs = caxpy(Bs, gl::Root<R, q>::s, v[R - m]);
+        });
+        const float2 iB = make_float2(-Bs.y, Bs.x);   // i * sum d_m sin(theta)
+        o[k] = DIR > 0 ? csub(A, iB) : cadd(A, iB);
+        o[R - k] = DIR > 0 ? cadd(A, iB) : csub(A, iB);
+    });
+#pragma unroll
+    for (int k = 0; k < R; ++k) v[k] = o[k];
+}
+
+template <int R, int DIR>
+__device__ __forceinline__ void dft_any(float2 (&v)[R]) {
+    if constexpr ((R & 1) == 0) Dft<R, DIR>::run(v);
+    else dft_odd<R, DIR>(v);
+}
+
+// One radix-R butterfly j of the stage (N, Ns): twiddle, DFT; returns the output base
+// (outputs at base + r Ns).  tw: W_N^m, m < N.
+template <int R, int DIR>
+__device__ __forceinline__ int gbutterfly(float2 (&v)[R], int j, int ns, int n,
+                                          const float2 *__restrict__ tw) {
+    const int k = j % ns;
+    if (ns > 1) {
+        const int step = k * (n / (ns * R));
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+            const float2 w = tw[step * r];
+            v[r] = DIR > 0 ? cmul(v[r], w) : cmulc(v[r], w);
+        }
+    }
+    dft_any<R, DIR>(v);
+    return (j / ns) * ns * R + k;
+}
+
+// Stage (N, Ns) of radix R on the wave's line, in place.  LAST (Ns R = N): every butterfly
+// writes the positions it read, so butterflies run one at a time; otherwise the lane's inputs
+// are all read (PM butterflies) before any output is written.
+template <int R, int DIR, bool LAST>
+__device__ __forceinline__ void gstage_body(float2 *line, int lane, int n, int ns,
+                                            const float2 *__restrict__ tw) {
+    const int nb = n / R;
+    if constexpr (LAST) {
+        for (int j = lane; j < nb; j += 64) {
+            float2 v[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[r] = line[pidx(j + r * nb)];
+            const int base = gbutterfly<R, DIR>(v, j, ns, n, tw);
+#pragma unroll
+            for (int r = 0; r < R; ++r) line[pidx(base + r * ns)] = v[r];
+        }
+    } else {
+        constexpr int PM = (gl::NTMAX / R + 63) / 64;
+        float2 v[PM][R];
+#pragma unroll
+        for (int p = 0; p < PM; ++p) {
+            const int j = lane + 64 * p;
+            if (j < nb) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[p][r] = line[pidx(j + r * nb)];
+            }
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int p = 0; p < PM; ++p) {
+            const int j = lane + 64 * p;
+            if (j < nb) {
+                const int base = gbutterfly<R, DIR>(v[p], j, ns, n, tw);
+#pragma unroll
+                for (int r = 0; r < R; ++r) line[pidx(base + r * ns)] = v[p][r];
+            }
+        }
+    }
+    wave_lds_sync();
+}
+
+// The earlier stages are out-of-line functions (one copy per radix and direction; inlined
+// into one switch they made the register allocator spill), the last stage is inlined.  The
+// stage functions use few enough registers that calls save none.
+template <int R, int DIR>
+__device__ __noinline__ void gstage_call(float2 *line, int lane, int n, int ns,
+                                         const float2 *__restrict__ tw) {
+    gstage_body<R, DIR, false>(line, lane, n, ns, tw);
+}
+template <int R, int DIR, bool LAST>
+__device__ __forceinline__ void gstage(float2 *line, int lane, int n, int ns,
+                                       const float2 *__restrict__ tw) {
+    if constexpr (LAST) gstage_body<R, DIR, true>(line, lane, n, ns, tw);
+    else gstage_call<R, DIR>(line, lane, n, ns, tw);
+}
+
+// Radix plan of an N_t (host-built): nst radices, ascending, product nt, five bits each in
+// rads (stage s at bits 5 s .. 5 s + 4).
+struct GPlan {
+    int nt, nst, c;   // c = ceil(nt / 64): time samples per lane
+    int pad;
+    unsigned long long rads;
+};
+
+// Radices 2, 3, 4, 5, 7, 8, 11, 13, 16 may take any stage; 17, 19, 23, 29, 31 only the last
+// one (make_gplan keeps at most one such factor), whose butterflies run one at a time.
+template <int DIR, bool LAST>
+__device__ __forceinline__ void gstage_r(int r, float2 *line, int lane, int n, int ns,
+                                         const float2 *__restrict__ tw) {
+    switch (r) {
+        case 2: gstage<2, DIR, LAST>(line, lane, n, ns, tw); break;
+        case 3: gstage<3, DIR, LAST>(line, lane, n, ns, tw); break;
+        case 4: gstage<4, DIR, LAST>(line, lane, n, ns, tw); break;
+        case 5: gstage<5, DIR, LAST>(line, lane, n, ns, tw); break;
+        case 7: gstage<7, DIR, LAST>(line, lane, n, ns, tw); break;
+        case 8: gstage<8, DIR, LAST>(line, lane, n, ns, tw); break;
+        case 11: gstage<11, DIR, LAST>(line, lane, n, ns, tw); break;
+        case 13: gstage<13, DIR, LAST>(line, lane, n, ns, tw); break;
+        case 16: gstage<16, DIR, LAST>(line, lane, n, ns, tw); break;
+        default:
+            if constexpr (LAST) {
+                switch (r) {
+                    case 17: gstage<17, DIR, true>(line, lane, n, ns, tw); break;
+                    case 19: gstage<19, DIR, true>(line, lane, n, ns, tw); break;
+                    case 23: gstage<23, DIR, true>(line, lane, n, ns, tw); break;
+                    case 29: gstage<29, DIR, true>(line, lane, n, ns, tw); break;
+                    default: gstage<31, DIR, true>(line, lane, n, ns, tw); break;
+                }
+            }
+            break;
+    }
+}
+
+// Unnormalised DFT of the wave's line (natural order in and out, at pidx positions).  The
+// caller synchronises the wave before (line written) and after (line read by other lanes).
+template <int DIR>
+__device__ __forceinline__ void gfft_n(float2 *line, int lane, int nt, int nst,
+                                    unsigned long long rads, const float2 *__restrict__ tw) {
+    int ns = 1;
+    for (int s = 0; s + 1 < nst; ++s) {
+        const int r = (int)((rads >> (5 * s)) & 31ull);
+        gstage_r<DIR, false>(r, line, lane, nt, ns, tw);
+        ns *= r;
+    }
+    gstage_r<DIR, true>((int)((rads >> (5 * (nst - 1))) & 31ull), line, lane, nt, ns, tw);
+}
+template <int DIR>
+__device__ __forceinline__ void gfft(float2 *line, int lane, const GPlan &gp,
+                                     const float2 *__restrict__ tw) {
+    gfft_n<DIR>(line, lane, gp.nt, gp.nst, gp.rads, tw);
+}
+
+// plasma_walk for a lane holding cnt (<= CM) consecutive samples t = t0 + k of its line (time
+// order = lane order; lanes past the end hold none).  Same recurrence, scan and fallback.
+template <class Sink>
+__device__ __forceinline__ void plasma_walk_g(float2 (&x)[gl::CM], int cnt, int lane,
+                                              const Phys3D &ph, const double *__restrict__ lut,
+                                              Sink sink) {
+    float wi[gl::CM];
+    bool risk = false;
+#pragma unroll
+    for (int k = 0; k < gl::CM; ++k) {
+        wi[k] = 0.0f;
+        if (k < cnt) {
+            const float I = ph.a_scale2 * cabs2(x[k]);
+            wi[k] = ((double)I > ph.lut_zero_below) ? lut_rate(I, lut, ph) : 0.0f;
+            risk |= ((double)wi[k] * ph.f_rec > ph.e_rec);
+        }
+    }
+    const bool aval = ph.av_coef != 0.0;
+    if (!aval && __ballot(risk) == 0) {
+        double A = 1.0, Bc = 0.0;
+#pragma unroll
+        for (int k = 0; k < gl::CM; ++k) {
+            if (k < cnt) {
+                const double w = (double)wi[k];
+                const double ak = ph.e_rec - w * ph.f_rec;
+                Bc = ak * Bc + w * ph.rho_nt * ph.f_rec;
+                A = ak * A;
+            }
+        }
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const double Ao = __shfl_up(A, off, 64);
+            const double Bo = __shfl_up(Bc, off, 64);
+            if (lane >= off) {
+                Bc = A * Bo + Bc;
+                A = A * Ao;
+            }
+        }
+        double rho = __shfl_up(Bc, 1, 64);
+        if (lane == 0) rho = 0.0;
+#pragma unroll
+        for (int k = 0; k < gl::CM; ++k) {
+            if (k < cnt) {
+                float c1, c2;
+                k1_affine(rho, (double)wi[k], (double)(ph.a_scale2 * cabs2(x[k])), ph, c1, c2);
+                sink(k, c1, c2);
+            }
+        }
+        return;
+    }
+    double carry = 0.0;
+#pragma unroll 1
+    for (int L = 0; L < 64; ++L) {
+        double mine = 0.0;
+        if (lane == L) {
+            double rho = carry;
+#pragma unroll 1
+            for (int k = 0; k < cnt; ++k) {
+                const float I = ph.a_scale2 * cabs2(pick<gl::CM>(x, k));
+                const double wav = aval ? (double)(float)(ph.av_coef * (double)I) : 0.0;
+                float c1, c2;
+                k1_general(rho, (double)pick<gl::CM>(wi, k), wav, (double)I, ph, c1, c2);
+#pragma unroll
+                for (int kk = 0; kk < gl::CM; ++kk)
+                    if (kk == k) sink(kk, c1, c2);
+            }
+            mine = rho;
+        }
+        carry = __shfl(mine, L, 64);
+    }
+}
+
+// t-pass on a t axis of any supported length: MODE 0 (out = Y), 1 (out = Y4 - Y) and 2
+// (out = Y), the final-stage modes also writing B' = X filter absorb to the spare state, as
+// tpass_group; the plasma term is evaluated in the same launch (no deferral).  A workgroup
+// takes 8 lines (fixed x, 8 consecutive y: 64-byte rows).  With both the shock operator and
+// plasma on a line, N = mask (shock FFT_t(Kerr) + FFT_t(Plasma)) needs two transforms: the
+// line's Kerr samples wait in f.X (dead between a t-pass reading it and the next xy pass
+// writing it; each wave touches only its own line), the plasma spectrum in `out`.
+__global__ __launch_bounds__(gl::T) void gtpass_kernel(Fields3D f, const float2 *in_fixed,
+                                                      int in_is_state, float2 *out, Tabs3D tb,
+                                                      Phys3D ph, Ctl3D *ctl, int nx, int ny,
+                                                      GPlan gp, int mode) {
+    __shared__ float2 lines[gl::LINES * gl::LP];
+    if (ctl->done) return;
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int n = gp.nt;
+    const int bpx = ny / gl::LINES;
+    const int x = blockIdx.x / bpx, y0 = (blockIdx.x % bpx) * gl::LINES;
+    const size_t plane = (size_t)nx * ny;
+    const size_t col0 = (size_t)x * ny + y0;
+    const int q = tid % gl::LINES, row0 = tid / gl::LINES;
+    const int cur = ctl->cur;
+    const float2 *in = in_is_state ? (cur ? f.B1 : f.B0) : in_fixed;
+    float absorb = 1.0f;
+    float2 *bnew = nullptr;
+    if (mode != 0) {
+        absorb = expf(-tb.damp[col0 + q] * (float)ctl->h);
+        bnew = cur ? f.B0 : f.B1;
+    }
+    for (int w = row0; w < n; w += gl::T / gl::LINES) {
+        const size_t e = (size_t)w * plane + col0 + q;
+        const float2 g = in[e];
+        if (bnew != nullptr) bnew[e] = cscale(g, tb.filt_w[w] * absorb);
+        lines[q * gl::LP + pidx(w)] = g;
+    }
+    __syncthreads();
+    float2 *line = lines + wv * gl::LP;
+    gfft<-1>(line, lane, gp, tb.tw_t);
+    const int t0 = lane * gp.c;
+    const int cnt = n - t0 < 0 ? 0 : (n - t0 > gp.c ? gp.c : n - t0);
+    const float sc = 1.0f / (float)n;
+    float2 xv[gl::CM];
+    float imax = 0.0f;
+#pragma unroll
+    for (int k = 0; k < gl::CM; ++k) {
+        xv[k] = make_float2(0.0f, 0.0f);
+        if (k < cnt) {
+            xv[k] = cscale(line[pidx(t0 + k)], sc);
+            imax = fmaxf(imax, ph.a_scale2 * cabs2(xv[k]));
+        }
+    }
+    imax = wave_max(imax);
+    const bool plasma = imax >= ctl->plasma_thr;
+    const bool two = plasma && ph.use_shock;
+    const size_t mine = col0 + wv;   // this wave's line: element (w, line) at w * plane + mine
+    if (!two) {
+        auto total = [&](int k, float c1, float c2) {
+            const float I = ph.a_scale2 * cabs2(xv[k]);
+            const float im = I * (ph.g_n2 + ph.g_n4 * I) - c2;
+            xv[k] = make_float2(-c1 * xv[k].x - im * xv[k].y, -c1 * xv[k].y + im * xv[k].x);
+        };
+        if (plasma) {
+            plasma_walk_g(xv, cnt, lane, ph, tb.lut, total);
+        } else {
+#pragma unroll
+            for (int k = 0; k < gl::CM; ++k)
+                if (k < cnt) total(k, 0.0f, 0.0f);
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int k = 0; k < gl::CM; ++k)
+            if (k < cnt) line[pidx(t0 + k)] = xv[k];
+    } else {
+        // plasma part P (time) -> LDS, transformed first; Kerr part K (time) -> f.X.  Each lane
+        // reads back only the elements it wrote (its time chunk in f.X, its omega stride in out).
+        auto split = [&](int k, float c1, float c2) {
+            line[pidx(t0 + k)] = make_float2(-c1 * xv[k].x + c2 * xv[k].y,
+                                             -c1 * xv[k].y - c2 * xv[k].x);
+        };
+        wave_lds_sync();
+        plasma_walk_g(xv, cnt, lane, ph, tb.lut, split);
+#pragma unroll
+        for (int k = 0; k < gl::CM; ++k) {
+            if (k < cnt) {
+                const float I = ph.a_scale2 * cabs2(xv[k]);
+                const float kap = I * (ph.g_n2 + ph.g_n4 * I);
+                f.X[(size_t)(t0 + k) * plane + mine] = make_float2(-kap * xv[k].y, kap * xv[k].x);
+            }
+        }
+    }
+    // forward transforms: one (Kerr [+ plasma]), or two (plasma, then Kerr)
+    for (int pass = 0;; ++pass) {
+        wave_lds_sync();
+        gfft<1>(line, lane, gp, tb.tw_t);
+        if (!two) {
+            const bool sh = ph.use_shock != 0;
+            for (int w = lane; w < n; w += 64) {
+                const float m = sh ? tb.shock_w[w] * tb.mask_w[w] : tb.mask_w[w];
+                line[pidx(w)] = cscale(line[pidx(w)], m);
+            }
+            break;
+        }
+        if (pass == 0) {
+            for (int w = lane; w < n; w += 64)
+                out[(size_t)w * plane + mine] = cscale(line[pidx(w)], tb.mask_w[w]);
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+            wave_lds_sync();
+#pragma unroll
+            for (int k = 0; k < gl::CM; ++k)
+                if (k < cnt) line[pidx(t0 + k)] = f.X[(size_t)(t0 + k) * plane + mine];
+            continue;
+        }
+        for (int w = lane; w < n; w += 64)
+            line[pidx(w)] = cadd(cscale(line[pidx(w)], tb.shock_w[w] * tb.mask_w[w]),
+                                 out[(size_t)w * plane + mine]);
+        break;
+    }
+    __syncthreads();
+    for (int w = row0; w < n; w += gl::T / gl::LINES) {
+        const size_t e = (size_t)w * plane + col0 + q;
+        float2 val = lines[q * gl::LP + pidx(w)];
+        if (mode == 1) val = csub(f.Y4[e], val);
+        out[e] = val;
+    }
+}
+
+// (x, y, t) input -> state, any supported N_t (load_kernel's job): one line per wavefront.
+__global__ __launch_bounds__(gl::T) void gload_kernel(const float2 *__restrict__ a0, Fields3D f,
+                                                     Tabs3D tb, Ctl3D *ctl, float inv_scale,
+                                                     int nx, int ny, GPlan gp) {
+    __shared__ float2 lines[gl::LINES * gl::LP];
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int n = gp.nt, bpx = ny / gl::LINES;
+    const int x = blockIdx.x / bpx, y0 = (blockIdx.x % bpx) * gl::LINES;
+    float2 *line = lines + wv * gl::LP;
+    const float2 *src = a0 + ((size_t)x * ny + y0 + wv) * n;
+    for (int t = lane; t < n; t += 64) line[pidx(t)] = cscale(src[t], inv_scale);
+    wave_lds_sync();
+    gfft<1>(line, lane, gp, tb.tw_t);
+    __syncthreads();
+    float2 *B = ctl->cur ? f.B1 : f.B0;
+    const size_t plane = (size_t)nx * ny, col0 = (size_t)x * ny + y0;
+    const int q = tid % gl::LINES;
+    for (int w = tid / gl::LINES; w < n; w += gl::T / gl::LINES)
+        B[(size_t)w * plane + col0 + q] = lines[q * gl::LP + pidx(w)];
+}
+
+// state -> (x, y, t) output, any supported N_t (store_kernel's job).  SPECTRUM: fftshift_t(B)
+// scale; else iFFT_t(B) scale.
+__global__ __launch_bounds__(gl::T) void gstore_kernel(float2 *__restrict__ out, Fields3D f,
+                                                      Tabs3D tb, Ctl3D *ctl, float scale, int nx,
+                                                      int ny, GPlan gp, int spectrum) {
+    __shared__ float2 lines[gl::LINES * gl::LP];
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int n = gp.nt, bpx = ny / gl::LINES;
+    const int x = blockIdx.x / bpx, y0 = (blockIdx.x % bpx) * gl::LINES;
+    const float2 *B = ctl->cur ? f.B1 : f.B0;
+    const size_t plane = (size_t)nx * ny, col0 = (size_t)x * ny + y0;
+    const int q = tid % gl::LINES;
+    for (int w = tid / gl::LINES; w < n; w += gl::T / gl::LINES)
+        lines[q * gl::LP + pidx(w)] = B[(size_t)w * plane + col0 + q];
+    __syncthreads();
+    float2 *line = lines + wv * gl::LP;
+    float2 *dst = out + ((size_t)x * ny + y0 + wv) * n;
+    if (spectrum) {
+        const int sh = n / 2;   // fftshift: output i holds input (i + n - n / 2) mod n
+        for (int i = lane; i < n; i += 64) {
+            int w = i + n - sh;
+            w = w >= n ? w - n : w;
+            dst[i] = cscale(line[pidx(w)], scale);
+        }
+        return;
+    }
+    gfft<-1>(line, lane, gp, tb.tw_t);
+    const float s = scale / (float)n;
+    for (int t = lane; t < n; t += 64) dst[t] = cscale(line[pidx(t)], s);
+}
+
+// Snapshot sums of diag_kernel for any supported N_t: two lines per workgroup (the FP64
+// accumulators take 6 N_t doubles of LDS).
+namespace gdg {
+constexpr int LINES = 2;
+constexpr int T = 64 * LINES;
+}
+__global__ __launch_bounds__(gdg::T) void gdiag_kernel(const Ctl3D *ctl, Fields3D f, Tabs3D tb,
+                                                      Diag3D dg, int nx, int ny, GPlan gp) {
+    __shared__ double acc[6 * gl::NTMAX];
+    __shared__ float2 lines[gdg::LINES * gl::LP];
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int n = gp.nt;
+    for (int i = tid; i < 6 * n; i += gdg::T) acc[i] = 0.0;
+    __syncthreads();
+    const float2 *B = ctl->cur ? f.B1 : f.B0;
+    const int plane = nx * ny, bpx = ny / gdg::LINES, groups = nx * bpx;
+    float2 *line = lines + wv * gl::LP;
+    for (int g = blockIdx.x; g < groups; g += gridDim.x) {
+        const int x = g / bpx, y = (g % bpx) * gdg::LINES + wv;
+        const double xd = dg.x0 + x * dg.dx, yd = dg.y0 + y * dg.dy;
+        const float m = (float)exp(-(xd * xd + yd * yd) * dg.inv_w02);
+        const bool centre = (x == dg.ix0 && y == dg.iy0);
+        for (int w = lane; w < n; w += 64) {
+            const float2 v = B[(size_t)w * plane + x * ny + y];
+            const double b2 = (double)v.x * v.x + (double)v.y * v.y;
+            atomicAdd(&acc[w], b2);
+            atomicAdd(&acc[n + w], (double)(v.x * m));
+            atomicAdd(&acc[2 * n + w], (double)(v.y * m));
+            if (centre) dg.center[w] = b2;
+            line[pidx(w)] = v;
+        }
+        wave_lds_sync();
+        gfft<-1>(line, lane, gp, tb.tw_t);
+        const float s = 1.0f / (float)n;
+        double fl = 0.0;
+        for (int t = lane; t < n; t += 64) {
+            const float2 v = cscale(line[pidx(t)], s);
+            const double a2 = (double)v.x * v.x + (double)v.y * v.y;
+            fl += a2;
+            atomicAdd(&acc[3 * n + t], a2);
+            atomicAdd(&acc[4 * n + t], (double)(v.x * m));
+            atomicAdd(&acc[5 * n + t], (double)(v.y * m));
+            if (centre) dg.center[n + t] = a2;
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) fl += __shfl_xor(fl, o, 64);
+        if (lane == 0) dg.fluence[x * ny + y] = fl;
+    }
+    __syncthreads();
+    double *outp = dg.part + (size_t)blockIdx.x * 6 * n;
+    for (int i = tid; i < 6 * n; i += gdg::T) outp[i] = acc[i];
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------------------
@@ -2991,6 +3521,8 @@ struct uppe3d_plan {
     // error pass can divide the filter and absorber out of B' (min over the grid of
     // filter * exp(-damp h) at least 1e-30: FP32 keeps full precision for every element
     // above 1e-8 of the field maximum)
+    GPlan gp;           // radix plan of a t axis that is not a power of two
+    int generic_t;      // 1: N_t is not a power of two (the gtpass / gload / gstore kernels)
     int fused_enabled;  // uppe3d_set_fused
     int fused_ok;       // grid and filter admit the restructured attempt
     int step_ok;        // min filter * exp(-max damp * h_max) >= 1e-30 for the current run
@@ -3049,13 +3581,51 @@ Layout3D layout3d(const uppe3d_desc &d) {
 
 bool pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 
+// Radix plan of a t axis for the generic kernels: 128 <= nt <= 2048, prime factors <= 31 with
+// at most one of them above 13; powers of two in radix-16 stages (remainder 8, 4 or 2),
+// radices ascending (16 before 17).  false otherwise.
+bool make_gplan(int nt, GPlan &gp) {
+    std::vector<int> r;
+    int m = nt;
+    int twos = 0;
+    while (m % 2 == 0) {
+        m /= 2;
+        ++twos;
+    }
+    int big = 0;   // factors 17..31: one at most, and it takes the last stage
+    for (int p = 3; p <= 31 && m > 1; p += 2)
+        while (m % p == 0) {
+            m /= p;
+            r.push_back(p);
+            if (p >= 17) ++big;
+        }
+    if (m != 1 || big > 1) return false;
+    while (twos >= 4) {
+        r.push_back(16);
+        twos -= 4;
+    }
+    if (twos > 0) r.push_back(1 << twos);
+    std::sort(r.begin(), r.end());
+    if ((int)r.size() > gl::MAXST) return false;
+    gp.nt = nt;
+    gp.nst = (int)r.size();
+    gp.c = (nt + 63) / 64;
+    gp.pad = 0;
+    gp.rads = 0ull;
+    for (int s = 0; s < gp.nst; ++s) gp.rads |= (unsigned long long)r[s] << (5 * s);
+    return true;
+}
+
 int check_desc(const uppe3d_desc *d) {
     UPPE_CHECK(d != nullptr, "uppe3d: null descriptor");
     UPPE_CHECK(d->nx == d->ny, "uppe3d: nx (%d) must equal ny (%d)", d->nx, d->ny);
     UPPE_CHECK(d->nx == 16 || d->nx == 32 || d->nx == 64 || d->nx == 128 || d->nx == 256,
                "uppe3d: nx = %d unsupported (16, 32, 64, 128, 256)", d->nx);
-    UPPE_CHECK(pow2(d->nt) && d->nt >= 128 && d->nt <= 2048,
-               "uppe3d: nt = %d unsupported (power of two in [128, 2048])", d->nt);
+    GPlan gp;
+    UPPE_CHECK(d->nt >= 128 && d->nt <= 2048 && (pow2(d->nt) || make_gplan(d->nt, gp)),
+               "uppe3d: nt = %d unsupported (128 <= nt <= 2048, prime factors at most 31, at "
+               "most one of them above 13)",
+               d->nt);
     UPPE_CHECK(d->lut_n >= 2, "uppe3d: lut_n must be >= 2");
     UPPE_CHECK(d->dt > 0.0 && d->a_scale > 0.0, "uppe3d: dt and a_scale must be positive");
     return UPPE_OK;
@@ -3088,6 +3658,12 @@ void twiddles(std::vector<float2> &v, int n) {
 int launch_tpass(uppe3d_plan *p, int mode, int slot, const float2 *in, int in_is_state,
                  float2 *out, hipStream_t s) {
     const int nx = p->d.nx, ny = p->d.ny;
+    if (p->generic_t) {
+        gtpass_kernel<<<nx * (ny / gl::LINES), gl::T, 0, s>>>(p->f, in, in_is_state, out, p->tb,
+                                                             p->ph, p->ctl, nx, ny, p->gp, mode);
+        UPPE_HIP(hipGetLastError());
+        return UPPE_OK;
+    }
     const int lines = p->d.nt >= 2048 ? 8 : 16;
     const int groups = nx * (ny / lines);
     // the plasma launch loops over its work list with a grid stride, so any grid is correct;
@@ -3518,6 +4094,9 @@ int uppe3d_plan_create(const uppe3d_desc *d, void *workspace, size_t workspace_b
     p->list = (int *)(ws + L.list);
     p->rec[0] = p->rec[1] = p->rec[2] = nullptr;
     p->plasma_small = 0;
+    p->generic_t = pow2(nt) ? 0 : 1;
+    memset(&p->gp, 0, sizeof p->gp);
+    if (p->generic_t) make_gplan(nt, p->gp);
     {
         double fmin = 1e300, dmax = 0.0;
         for (int i = 0; i < nt; ++i) fmin = filt_w[i] < fmin ? filt_w[i] : fmin;
@@ -3527,7 +4106,8 @@ int uppe3d_plan_create(const uppe3d_desc *d, void *workspace, size_t workspace_b
         p->damp_max = (double)(float)dmax;
         // N_t 2048 keeps the classic attempt: three line arrays of 32 values per lane exceed
         // the register file of a fused t-pass
-        p->fused_ok = (nx <= 128 && nt <= 1024 && p->filt_min > 0.0 && std::isfinite(dmax)) ? 1 : 0;
+        p->fused_ok = (nx <= 128 && nt <= 1024 && pow2(nt) && p->filt_min > 0.0 &&
+                       std::isfinite(dmax)) ? 1 : 0;
         p->fused_enabled = 1;
         p->step_ok = 0;
     }
@@ -3622,6 +4202,12 @@ int uppe3d_load_field(uppe3d_plan *p, const void *a0, void *stream) {
     hipStream_t s = (hipStream_t)stream;
     const int nx = p->d.nx, ny = p->d.ny;
     const float inv = (float)(1.0 / p->d.a_scale);
+    if (p->generic_t) {
+        gload_kernel<<<nx * (ny / gl::LINES), gl::T, 0, s>>>((const float2 *)a0, p->f, p->tb,
+                                                            p->ctl, inv, nx, ny, p->gp);
+        UPPE_HIP(hipGetLastError());
+        return UPPE_OK;
+    }
     switch (p->d.nt) {
 #define LOAD(NT, LN)                                                                      \
     case NT:                                                                              \
@@ -3646,6 +4232,12 @@ static int store_common(uppe3d_plan *p, void *out, void *stream, bool spectrum) 
     hipStream_t s = (hipStream_t)stream;
     const int nx = p->d.nx, ny = p->d.ny;
     const float sc = (float)p->d.a_scale;
+    if (p->generic_t) {
+        gstore_kernel<<<nx * (ny / gl::LINES), gl::T, 0, s>>>((float2 *)out, p->f, p->tb, p->ctl,
+                                                             sc, nx, ny, p->gp, spectrum ? 1 : 0);
+        UPPE_HIP(hipGetLastError());
+        return UPPE_OK;
+    }
     switch (p->d.nt) {
 #define STORE(NT, LN)                                                                         \
     case NT:                                                                                  \
@@ -3804,7 +4396,8 @@ int uppe3d_get_state(uppe3d_plan *p, void *dst, void *stream) {
 }
 
 static int diag_parts(const uppe3d_desc &d) {
-    const int lines = d.nt >= 1024 ? 8 : 16;   // as the DIAG launches below
+    // as the DIAG launches below (gdiag_kernel: two lines per workgroup)
+    const int lines = !pow2(d.nt) ? gdg::LINES : (d.nt >= 1024 ? 8 : 16);
     const int groups = d.nx * (d.ny / lines);
     return groups < 256 ? groups : 256;
 }
@@ -3834,6 +4427,9 @@ int uppe3d_diagnostics(uppe3d_plan *p, double w0_mode, double x0, double dx, dou
     dg.part = (double *)work;
     dg.center = o + 6 * (size_t)nt;
     dg.fluence = o + 8 * (size_t)nt;
+    if (p->generic_t) {
+        gdiag_kernel<<<G, gdg::T, 0, s>>>(p->ctl, p->f, p->tb, dg, nx, ny, p->gp);
+    } else {
     switch (nt) {
 #define DIAG(NT, LN)                                                                     \
     case NT:                                                                             \
@@ -3847,6 +4443,7 @@ int uppe3d_diagnostics(uppe3d_plan *p, double w0_mode, double x0, double dx, dou
 #undef DIAG
         default:
             return UPPE_EINVAL;
+    }
     }
     UPPE_HIP(hipGetLastError());
     diag_reduce_kernel<<<(6 * nt + 255) / 256, 256, 0, s>>>(dg.part, G, 6 * nt, o);
@@ -3897,6 +4494,8 @@ int uppe3d_set_stage_record(uppe3d_plan *p, void *x2, void *x3, void *x4) {
 int uppe3d_adjoint_workspace_size(const uppe3d_desc *d, size_t *bytes) {
     RET(check_desc(d));
     UPPE_CHECK(bytes != nullptr, "uppe3d_adjoint_workspace_size: null output");
+    UPPE_CHECK(pow2(d->nt), "uppe3d adjoint: nt = %d is not a power of two (the adjoint "
+                            "t-pass takes 128, 256, ..., 2048)", d->nt);
     *bytes = 5 * align_up(field_bytes(*d));
     return UPPE_OK;
 }
@@ -3995,6 +4594,8 @@ int uppe3d_adjoint_step(uppe3d_plan *p, double h, const void *B, const void *X2,
     UPPE_CHECK(p && B && X2 && X3 && X4 && Bbar && adj_ws && h > 0.0,
                "uppe3d_adjoint_step: bad argument");
     UPPE_CHECK(((uintptr_t)adj_ws % kAlign) == 0, "uppe3d_adjoint_step: workspace not 256-B aligned");
+    UPPE_CHECK(!p->generic_t, "uppe3d_adjoint_step: nt = %d is not a power of two (the adjoint "
+                              "t-pass takes 128, 256, ..., 2048)", p->d.nt);
     hipStream_t s = (hipStream_t)stream;
     const size_t fb = align_up(field_bytes(p->d));
     char *w = (char *)adj_ws;

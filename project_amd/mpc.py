@@ -16,8 +16,9 @@ Interface kept: ``MPC_Final_Runner()`` with the reference's physical constants
 (the ``cycle_XX_<label>_DIAG.npz`` keys of ``:245-249``; the reference's figure, ``:251-302``,
 is not drawn: every quantity it shows is in the npz) and ``run``.  Fields are normalised by the runner's ``A_scale``
 (``:98-118``) as in the reference, as torch complex64 tensors on the GPU.
-Differences: N_t must be a power of two from 128 to 2048 (default 2048; the reference uses
-2046, which the HIP t transforms do not take); ``run`` returns the per-cycle log.
+Differences: N_t from 128 to 2048 with prime factors of at most 31 (the runner's own 2046 =
+2 x 3 x 11 x 31, the default, runs on mixed-radix line transforms; powers of two on the faster
+register transforms); ``run`` returns the per-cycle log.
 """
 import os
 import time
@@ -37,7 +38,7 @@ class MPC_Final_Runner:
         L_cavity=212.5e-3, R_concave=300e-3,
         d_fs=12e-3, d_air_1=30e-3, d_air_mid=25e-3, n_plates=4,
         cycles=33, total_trans=53.9 / 100.0,
-        window_width=2.0e-3, Nxy=256, Nt=2048, time_window=80e-12,
+        window_width=2.0e-3, Nxy=256, Nt=2046, time_window=80e-12,
         n2_fs=3.0e-20, verbose=True)
 
     def __init__(self, **overrides):

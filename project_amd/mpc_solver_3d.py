@@ -15,8 +15,8 @@ Constructor kept: ``(x_grid, y_grid, t_grid, material='fused_silica', n2=3e-20,
 center_wavelength=800e-9, use_raman=True, use_shock=True, f_R=0.18, tau1=12.2e-15,
 tau2=32e-15, precision='single')``.  Differences: the Raman response (``:187-200,205-210``) is
 not implemented and ``use_raman=True`` raises (the runner passes ``use_raman=False``,
-``sim.py:61-68``); the t axis must be a power of two from 128 to 2048 (the runner's own grid is
-N_t = 2046).
+``sim.py:61-68``); the t axis takes 128 to 2048 points with prime factors of at most 31 (the
+runner's own N_t = 2046 included).
 """
 import numpy as np
 

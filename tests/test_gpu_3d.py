@@ -57,6 +57,13 @@ CASES = [
     # density recurrence, avalanche rate sigma/Ui I), NT = 256 and NT = 1024
     ("fs 32x256 avalanche", 32, 256, 0.6e-3, False, False, 2e-4, 1e-5, 1e-5),
     ("fs 32x1024 avalanche+shock", 32, 1024, 0.6e-3, False, True, 2e-4, 1e-5, 1e-5),
+    # t axes that are not a power of two (mixed-radix line transforms in LDS): the MPC
+    # runner's 2046 = 2 3 11 31, 1001 = 7 11 13 (odd: fftshift and chunk edges), 250 = 2 5^3 with
+    # the plasma scan, and the serial recurrence with shock (two forward transforms per line)
+    ("16x2046 kerr+hoke", 16, 2046, 0.8e-3, True, False, 0.02, 1e-5, 1e-5),
+    ("16x1001 kerr+hoke+shock", 16, 1001, 0.8e-3, True, True, 0.01, 1e-5, 1e-5),
+    ("32x250 plasma scan", 32, 250, 4e-3, False, False, 0.005, 1e-5, 1e-5),
+    ("32x1536 plasma stress", 32, 1536, 30e-3, False, True, 0.001, 1e-4, 1e-5),
 ]
 
 
@@ -72,7 +79,7 @@ def test_replay_parity(gpu, case):
     errs_o = np.array([l[1] for l in log])
     big = errs_o > 1e-7   # below that the FP32 norm is rounding-dominated
     # both attempt dataflows where the plan admits the restructured one (nx <= 128, N_t <= 1024)
-    expect = "restructured" if nx <= 128 and nt <= 1024 else "classic"
+    expect = "restructured" if nx <= 128 and nt <= 1024 and nt & (nt - 1) == 0 else "classic"
     res = {}
     for restructured in ((True, False) if expect == "restructured" else (True,)):
         solver.restructured = restructured

@@ -26,7 +26,7 @@ def field(nx, nt, seed):
     return x, y, t, A.astype(np.complex64)
 
 
-@pytest.mark.parametrize("nx,nt", [(32, 256), (16, 1024), (64, 128)])
+@pytest.mark.parametrize("nx,nt", [(32, 256), (16, 1024), (64, 128), (16, 2046), (32, 1001)])
 def test_snapshot_vs_oracle(gpu, nx, nt):
     x, y, t, A = field(nx, nt, 7)
     s = ERK43IP_UPPE_3D_Optimized(x, y, t, mp.Argon(800e-9, 1.0, tunnel_guard=0.0))
@@ -49,11 +49,17 @@ def test_snapshot_vs_oracle(gpu, nx, nt):
     assert g["fwhm_smf"] == pytest.approx(o["fwhm_smf"], rel=1e-5)
 
 
-def test_mirror_vs_oracle(gpu):
-    x, y, t, A = field(32, 256, 3)
+@pytest.mark.parametrize("nx,nt", [(32, 256), (16, 1001)])
+def test_mirror_vs_oracle(gpu, nx, nt):
+    x, y, t, A = field(nx, nt, 3)
     s = ERK43IP_UPPE_3D_Optimized(x, y, t, mp.Argon(800e-9, 1.0, tunnel_guard=0.0))
-    announce("mirror 32x32x256")
+    announce("mirror %dx%dx%d" % (nx, nx, nt))
     s.load(A)
+    # load (forward t transform) and store (inverse) round trip
+    back = s.field().cpu().numpy()
+    rt = np.linalg.norm(back - A) / np.linalg.norm(A)
+    print("load/store round trip rel L2", rt)
+    assert rt < 1e-6
     amp = np.sqrt(0.539 ** (1 / 33))
     dg.apply_mirror(s, 0.3, amp)
     got = s.field().cpu().numpy().astype(np.complex128)

@@ -100,12 +100,14 @@ def small_runner(tmp_path, **kw):
     return pm.MPC_Final_Runner(**cfg)
 
 
-def test_runner_cycle_vs_oracle(gpu, tmp_path):
-    r = small_runner(tmp_path)
+@pytest.mark.parametrize("nxy,nt", [(32, 256), (16, 2046)], ids=["32x256", "16x2046"])
+def test_runner_cycle_vs_oracle(gpu, tmp_path, nxy, nt):
+    """32 x 32 x 256, and the runner's own t axis N_t = 2046 (sim.py:48) on a 16 x 16 plane."""
+    r = small_runner(tmp_path, Nxy=nxy, Nt=nt)
     A = r.generate_initial_field()
     A_in = A.cpu().numpy().astype(np.complex128)
     seq = r.segments()
-    announce("mpc runner one cycle 32x32x256 segments=%d" % len(seq))
+    announce("mpc runner one cycle %dx%dx%d segments=%d" % (nxy, nxy, nt, len(seq)))
     out = r.cycle(A).cpu().numpy()
     s = r.solver_fs
     solvers = {"air": om.OracleMPC3D(s.x, s.y, s.t, om.SellmeierKerr("air", 0.0, LAM, False),
