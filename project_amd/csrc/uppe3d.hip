@@ -3198,6 +3198,80 @@ __device__ __forceinline__ void gfft(float2 *line, int lane, const GPlan &gp,
     gfft_n<DIR>(line, lane, gp.nt, gp.nst, gp.rads, tw);
 }
 
+// The same transform with the length and radix sequence fixed at compile time (the MPC
+// runner's N_t = 2046 = 2 3 11 31): stage strides are constants, so the butterfly index
+// arithmetic needs no runtime division, and every stage is inlined with LDS addressing.
+template <int R, int DIR, bool LAST, int N, int NS>
+__device__ __forceinline__ void gstage_ct(float2 *line, int lane,
+                                          const float2 *__restrict__ tw) {
+    constexpr int NB = N / R;
+    if constexpr (LAST) {
+        static_assert(NS * R == N, "the last stage writes the positions it read");
+#pragma unroll 1
+        for (int p = 0; p < (NB + 63) / 64; ++p) {
+            const int j = lane + 64 * p;
+            if (NB % 64 == 0 || j < NB) {
+                float2 v[R];
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[r] = line[pidx(j + r * NB)];
+                const int base = gbutterfly<R, DIR>(v, j, NS, N, tw);
+#pragma unroll
+                for (int r = 0; r < R; ++r) line[pidx(base + r * NS)] = v[r];
+            }
+        }
+    } else {
+        constexpr int PM = (NB + 63) / 64;
+        float2 v[PM][R];
+#pragma unroll
+        for (int p = 0; p < PM; ++p) {
+            const int j = lane + 64 * p;
+            if (NB % 64 == 0 || j < NB) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[p][r] = line[pidx(j + r * NB)];
+            }
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int p = 0; p < PM; ++p) {
+            const int j = lane + 64 * p;
+            if (NB % 64 == 0 || j < NB) {
+                const int base = gbutterfly<R, DIR>(v[p], j, NS, N, tw);
+#pragma unroll
+                for (int r = 0; r < R; ++r) line[pidx(base + r * NS)] = v[p][r];
+            }
+        }
+    }
+    wave_lds_sync();
+}
+
+// earlier stages out of line (as gstage_call), the last one inlined
+template <int R, int DIR, int N, int NS>
+__device__ __noinline__ void gstage_ct_call(float2 *line, int lane,
+                                            const float2 *__restrict__ tw) {
+    gstage_ct<R, DIR, false, N, NS>(line, lane, tw);
+}
+
+template <int N, int NS, int DIR, int R, int... RS>
+struct GSeq {
+    __device__ __forceinline__ static void run(float2 *line, int lane,
+                                               const float2 *__restrict__ tw) {
+        if constexpr (sizeof...(RS) > 0) {
+            gstage_ct_call<R, DIR, N, NS>(line, lane, tw);
+            GSeq<N, NS * R, DIR, RS...>::run(line, lane, tw);
+        } else {
+            gstage_ct<R, DIR, true, N, NS>(line, lane, tw);
+        }
+    }
+};
+
+// SPEC = 2046: the compile-time plan; SPEC = 0: the plan gp at run time
+template <int SPEC, int DIR>
+__device__ __forceinline__ void gfft_sel(float2 *line, int lane, const GPlan &gp,
+                                         const float2 *__restrict__ tw) {
+    if constexpr (SPEC == 2046) GSeq<2046, 1, DIR, 2, 3, 11, 31>::run(line, lane, tw);
+    else gfft<DIR>(line, lane, gp, tw);
+}
+
 // plasma_walk for a lane holding cnt (<= CM) consecutive samples t = t0 + k of its line (time
 // order = lane order; lanes past the end hold none).  Same recurrence, scan and fallback.
 template <class Sink>
@@ -3277,6 +3351,7 @@ __device__ __forceinline__ void plasma_walk_g(float2 (&x)[gl::CM], int cnt, int 
 // plasma on a line, N = mask (shock FFT_t(Kerr) + FFT_t(Plasma)) needs two transforms: the
 // line's Kerr samples wait in f.X (dead between a t-pass reading it and the next xy pass
 // writing it; each wave touches only its own line), the plasma spectrum in `out`.
+template <int SPEC>
 __global__ __launch_bounds__(gl::T) void gtpass_kernel(Fields3D f, const float2 *in_fixed,
                                                       int in_is_state, float2 *out, Tabs3D tb,
                                                       Phys3D ph, Ctl3D *ctl, int nx, int ny,
@@ -3306,7 +3381,7 @@ __global__ __launch_bounds__(gl::T) void gtpass_kernel(Fields3D f, const float2 
     }
     __syncthreads();
     float2 *line = lines + wv * gl::LP;
-    gfft<-1>(line, lane, gp, tb.tw_t);
+    gfft_sel<SPEC, -1>(line, lane, gp, tb.tw_t);
     const int t0 = lane * gp.c;
     const int cnt = n - t0 < 0 ? 0 : (n - t0 > gp.c ? gp.c : n - t0);
     const float sc = 1.0f / (float)n;
@@ -3362,7 +3437,7 @@ __global__ __launch_bounds__(gl::T) void gtpass_kernel(Fields3D f, const float2 
     // forward transforms: one (Kerr [+ plasma]), or two (plasma, then Kerr)
     for (int pass = 0;; ++pass) {
         wave_lds_sync();
-        gfft<1>(line, lane, gp, tb.tw_t);
+        gfft_sel<SPEC, 1>(line, lane, gp, tb.tw_t);
         if (!two) {
             const bool sh = ph.use_shock != 0;
             for (int w = lane; w < n; w += 64) {
@@ -3659,8 +3734,12 @@ int launch_tpass(uppe3d_plan *p, int mode, int slot, const float2 *in, int in_is
                  float2 *out, hipStream_t s) {
     const int nx = p->d.nx, ny = p->d.ny;
     if (p->generic_t) {
-        gtpass_kernel<<<nx * (ny / gl::LINES), gl::T, 0, s>>>(p->f, in, in_is_state, out, p->tb,
-                                                             p->ph, p->ctl, nx, ny, p->gp, mode);
+        if (p->d.nt == 2046)
+            gtpass_kernel<2046><<<nx * (ny / gl::LINES), gl::T, 0, s>>>(
+                p->f, in, in_is_state, out, p->tb, p->ph, p->ctl, nx, ny, p->gp, mode);
+        else
+            gtpass_kernel<0><<<nx * (ny / gl::LINES), gl::T, 0, s>>>(
+                p->f, in, in_is_state, out, p->tb, p->ph, p->ctl, nx, ny, p->gp, mode);
         UPPE_HIP(hipGetLastError());
         return UPPE_OK;
     }

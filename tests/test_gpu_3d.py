@@ -51,6 +51,8 @@ CASES = [
     ("16x1024 kerr+hoke", 16, 1024, 0.8e-3, True, False, 0.02, 1e-5, 1e-5),
     ("32x1024 plasma scan+shock", 32, 1024, 4e-3, False, True, 0.005, 1e-5, 1e-5),
     ("32x1024 plasma stress", 32, 1024, 30e-3, False, True, 0.001, 1e-4, 1e-5),
+    # 128 x 128 planes on a short t axis: the persistent error pass that ends the attempt
+    ("128x128 kerr+hoke+shock", 128, 128, 0.8e-3, True, True, 0.02, 1e-5, 1e-5),
     # planes beyond one workgroup's LDS (column pass + row kernel)
     ("256x128 kerr+hoke+shock", 256, 128, 0.8e-3, True, True, 0.02, 1e-5, 1e-5),
     # fused silica, peak 1e17 W/m^2: multiphoton and avalanche ionisation (the general FP64
