@@ -66,6 +66,8 @@ CASES = [
     ("16x1001 kerr+hoke+shock", 16, 1001, 0.8e-3, True, True, 0.01, 1e-5, 1e-5),
     ("32x250 plasma scan", 32, 250, 4e-3, False, False, 0.005, 1e-5, 1e-5),
     ("32x1536 plasma stress", 32, 1536, 30e-3, False, True, 0.001, 1e-4, 1e-5),
+    # the compile-time 2046 plan with the plasma scan and shock (two forward transforms)
+    ("32x2046 plasma scan+shock", 32, 2046, 4e-3, False, True, 0.005, 1e-5, 1e-5),
 ]
 
 
