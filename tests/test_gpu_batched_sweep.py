@@ -12,12 +12,14 @@ oracle, separating the two sources of difference:
   (e_cond, oracle replay from the rounded input).  The FP32 path rounds the state at each of
   its M steps and no later rounding is amplified more than the input's, so the bound is
   M e_cond (triangle inequality), and 2e-5 for well-conditioned instances;
-* step control: the free-running attempt count agrees within 1% where e_cond <= 1e-6.  Where
-  the problem is ill-conditioned the accept/reject decisions of attempts whose error lies near
-  tol follow the rounding, and the trajectory itself is only defined to the truncation error
-  (2% between tol and tol/10 on the hardest instance): the oracle's own algorithm in FP32
-  arithmetic moves these counts by 20-40% (403 -> 490 and 540 -> 326 attempts on instances
-  253 and 80), so there the counts are reported, not bounded.  The
+* step control: the free-running attempt count agrees within 1% where the count itself is
+  well defined, i.e. where the oracle's own algorithm in complex64 arithmetic (kind "free32")
+  lands within 1% of the FP64 count and e_cond <= 1e-6.  Where the problem is ill-conditioned
+  the accept/reject decisions of attempts whose error lies near tol follow the rounding, and
+  the trajectory itself is only defined to the truncation error (2% between tol and tol/10 on
+  the hardest instance): FP32 arithmetic moves these counts by 20-40% (403 -> 490 and
+  540 -> 326 attempts on instances 253 and 80, the first with e_cond = 4e-7), so there the
+  counts are reported, not bounded.  The
   free-running spectra differ by less than 3 times the oracle's own truncation error at this
   tolerance (its distance to the oracle at tol/10): two step sequences differing in the last
   digits of h give solutions that differ by about that."""
@@ -60,7 +62,7 @@ def test_full_sweep_subset_vs_oracle(gpu):
         pick.append(int(idx[np.argmax(att[idx])]))
     pick = list(dict.fromkeys(pick))
     announce("oracle: %d instances (free, tol/10, replay, rounded-input replay)" % len(pick))
-    kinds = ("free", "free10", "replay", "replay_c64_input")
+    kinds = ("free", "free10", "replay", "replay_c64_input", "free32")
     jobs = [(kind, params[k], inputs[k][0], inputs[k][1], L, tol, hs[k])
             for k in pick for kind in kinds]
     # longest solves first (the N_t = 8192 critical-path instance dominates)
@@ -73,6 +75,7 @@ def test_full_sweep_subset_vs_oracle(gpu):
     for i in range(len(pick)):
         r = {kind: done[i * len(kinds) + n][0] for n, kind in enumerate(kinds)}
         r["attempts"] = done[i * len(kinds)][1]
+        r["attempts32"] = done[i * len(kinds) + kinds.index("free32")][1]
         res.append(r)
     for k, r in zip(pick, res):
         e_rep = rel(spec[k], r["replay"])
@@ -80,10 +83,13 @@ def test_full_sweep_subset_vs_oracle(gpu):
         e_free = rel(spec[k], r["free"])
         e_trunc = rel(r["free10"], r["free"])
         b_rep = max(2e-5, len(hs[k]) * e_cond)
-        b_att = max(2, 0.01 * r["attempts"]) if e_cond <= 1e-6 else None
-        print("instance %3d nt %4d | attempts gpu %5d oracle %5d | replay %.2e (cond %.2e, "
-              "bound %.2e) | free %.2e (trunc %.2e)"
-              % (k, nts[k], att[k], r["attempts"], e_rep, e_cond, b_rep, e_free, e_trunc))
+        b_att = max(2, 0.01 * r["attempts"])
+        if e_cond > 1e-6 or abs(r["attempts32"] - r["attempts"]) > b_att:
+            b_att = None   # the count is not defined to 1% by the algorithm itself
+        print("instance %3d nt %4d | attempts gpu %5d oracle %5d (fp32 %5d) | replay %.2e "
+              "(cond %.2e, bound %.2e) | free %.2e (trunc %.2e)"
+              % (k, nts[k], att[k], r["attempts"], r["attempts32"], e_rep, e_cond, b_rep,
+                 e_free, e_trunc))
         assert e_rep < b_rep, (k, e_rep, b_rep)
         if b_att is not None:
             assert abs(att[k] - r["attempts"]) <= b_att, k
