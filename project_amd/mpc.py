@@ -202,10 +202,13 @@ class MPC_Final_Runner:
             A_norm = self.cycle(A_norm)
             torch.cuda.synchronize()
             elapsed = time.time() - t0   # the cycle itself, as sim.py:337 (not the diagnostics)
+            # sim.py:341: a NaN anywhere in the cycle's field (checked before the snapshot,
+            # which loads a cleaned copy when the field is not a solver's state)
+            field_nan = bool(torch.isnan(A_norm).any())
             s = self._held(A_norm)
             snap = _dg.snapshot(s, self.w0_waist)
             flu = snap["fluence"]
-            has_nan = bool(np.isnan(flu).any() or not np.isfinite(snap["energy_total"]))
+            has_nan = field_nan or bool(np.isnan(flu).any() or not np.isfinite(snap["energy_total"]))
             max_flu = float(np.nanmax(flu)) if flu.size else 0.0
             self._log("Cycle %-2d | Time: %.1fs | MaxFluence: %.2e J/m^2 | Status: %s"
                       % (i, elapsed, max_flu, "WARNING: NaN" if has_nan else "OK"))
