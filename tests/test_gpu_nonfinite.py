@@ -69,3 +69,33 @@ def test_mpc_runner_nan_guards(gpu, tmp_path):
     hist = r.run(cycles=3, snapshots=False)
     assert [h["cycle"] for h in hist] == [1, 2]
     assert [h["nan"] for h in hist] == [False, True]
+
+
+def test_3d_keyboard_interrupt_returns_partial_field(gpu):
+    """solver_3d.py:210-211: Ctrl-C during propagate returns the field reached so far.  The
+    device loop runs in bounded chunks of attempts, so the interrupt lands between chunks."""
+    import os
+    import signal
+    import threading
+    import time
+    x, y, t = wl.grid3d(32, 256)
+    A0 = wl.beam3d(x, y, t)
+    s = ERK43IP_UPPE_3D_Optimized(x, y, t, mp.Argon(800e-9, 1.0, tunnel_guard=0.0))
+    s.propagate(A0, 1e-4, tol=1e-5)          # warm-up (plan, code objects)
+    L = 10.0                                  # ~10^4 attempts at max_step 1 mm: seconds
+    timer = threading.Timer(0.5, lambda: os.kill(os.getpid(), signal.SIGINT))
+    announce("3d propagate interrupted by SIGINT after 0.5 s")
+    t0 = time.perf_counter()
+    timer.start()
+    try:
+        z, f, Aw = s.propagate(A0, L, tol=1e-5, max_step=1e-3)
+    except KeyboardInterrupt:   # escaped the solver: a failure of this test, not of the session
+        pytest.fail("KeyboardInterrupt escaped propagate")
+    finally:
+        timer.cancel()
+    wall = time.perf_counter() - t0
+    st = s.last_status
+    print("interrupted at z = %.4f m after %.2f s, %d attempts" % (z, wall, st.attempts))
+    assert 0.0 < z < L and not st.done
+    assert wall < 5.0
+    assert np.all(np.isfinite(Aw)) and Aw.shape == A0.shape
