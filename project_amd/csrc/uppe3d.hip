@@ -3361,7 +3361,9 @@ __global__ __launch_bounds__(gl::T) void gtpass_kernel(Fields3D f, const float2 
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int n = gp.nt;
     const int bpx = ny / gl::LINES;
-    const int x = blockIdx.x / bpx, y0 = (blockIdx.x % bpx) * gl::LINES;
+    // the two 8-line halves of a 128-byte row on one XCD (one L2 fetches the row once)
+    const int blk = xcd_pair_group(blockIdx.x, gridDim.x);
+    const int x = blk / bpx, y0 = (blk % bpx) * gl::LINES;
     const size_t plane = (size_t)nx * ny;
     const size_t col0 = (size_t)x * ny + y0;
     const int q = tid % gl::LINES, row0 = tid / gl::LINES;
@@ -3477,7 +3479,8 @@ __global__ __launch_bounds__(gl::T) void gload_kernel(const float2 *__restrict__
     __shared__ float2 lines[gl::LINES * gl::LP];
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int n = gp.nt, bpx = ny / gl::LINES;
-    const int x = blockIdx.x / bpx, y0 = (blockIdx.x % bpx) * gl::LINES;
+    const int blk = xcd_pair_group(blockIdx.x, gridDim.x);
+    const int x = blk / bpx, y0 = (blk % bpx) * gl::LINES;
     float2 *line = lines + wv * gl::LP;
     const float2 *src = a0 + ((size_t)x * ny + y0 + wv) * n;
     for (int t = lane; t < n; t += 64) line[pidx(t)] = cscale(src[t], inv_scale);
@@ -3499,7 +3502,8 @@ __global__ __launch_bounds__(gl::T) void gstore_kernel(float2 *__restrict__ out,
     __shared__ float2 lines[gl::LINES * gl::LP];
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int n = gp.nt, bpx = ny / gl::LINES;
-    const int x = blockIdx.x / bpx, y0 = (blockIdx.x % bpx) * gl::LINES;
+    const int blk = xcd_pair_group(blockIdx.x, gridDim.x);
+    const int x = blk / bpx, y0 = (blk % bpx) * gl::LINES;
     const float2 *B = ctl->cur ? f.B1 : f.B0;
     const size_t plane = (size_t)nx * ny, col0 = (size_t)x * ny + y0;
     const int q = tid % gl::LINES;
