@@ -3354,92 +3354,108 @@ __device__ __forceinline__ void plasma_walk_g(float2 (&x)[gl::CM], int cnt, int 
 template <int SPEC>
 __global__ __launch_bounds__(gl::T) void gtpass_kernel(Fields3D f, const float2 *in_fixed,
                                                       int in_is_state, float2 *out, Tabs3D tb,
-                                                      Phys3D ph, Ctl3D *ctl, int nx, int ny,
-                                                      GPlan gp, int mode) {
+                                                      Phys3D ph, const Phys3D *phd, Ctl3D *ctl,
+                                                      int nx, int ny, GPlan gp, int mode) {
     __shared__ float2 lines[gl::LINES * gl::LP];
     if (ctl->done) return;
-    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    // ph by value holds the hot scalars; the plasma code takes the device copy *phd by
+    // reference (a reference to the by-value argument would copy it to every lane's scratch)
     const int n = gp.nt;
     const int bpx = ny / gl::LINES;
-    // the two 8-line halves of a 128-byte row on one XCD (one L2 fetches the row once)
-    const int blk = xcd_pair_group(blockIdx.x, gridDim.x);
-    const int x = blk / bpx, y0 = (blk % bpx) * gl::LINES;
     const size_t plane = (size_t)nx * ny;
-    const size_t col0 = (size_t)x * ny + y0;
-    const int q = tid % gl::LINES, row0 = tid / gl::LINES;
-    const int cur = ctl->cur;
-    const float2 *in = in_is_state ? (cur ? f.B1 : f.B0) : in_fixed;
-    float absorb = 1.0f;
-    float2 *bnew = nullptr;
-    if (mode != 0) {
-        absorb = expf(-tb.damp[col0 + q] * (float)ctl->h);
-        bnew = cur ? f.B0 : f.B1;
-    }
-    for (int w = row0; w < n; w += gl::T / gl::LINES) {
-        const size_t e = (size_t)w * plane + col0 + q;
-        const float2 g = in[e];
-        if (bnew != nullptr) bnew[e] = cscale(g, tb.filt_w[w] * absorb);
-        lines[q * gl::LP + pidx(w)] = g;
+    // Per-lane indices are recomputed from an opaque copy of the thread index after each line
+    // transform (out-of-line stage calls): values held across a call are saved to scratch.
+    // The two 8-line halves of a 128-byte row go to one XCD (one L2 fetches the row once).
+    auto col0_of = [&]() {
+        const int blk = xcd_pair_group(blockIdx.x, gridDim.x);
+        return (size_t)(blk / bpx) * ny + (size_t)((blk % bpx) * gl::LINES);
+    };
+    {
+        const int tid = opaque_i(threadIdx.x);
+        const int q = tid % gl::LINES, row0 = tid / gl::LINES;
+        const size_t col0 = col0_of();
+        const int cur = ctl->cur;
+        const float2 *in = in_is_state ? (cur ? f.B1 : f.B0) : in_fixed;
+        float absorb = 1.0f;
+        float2 *bnew = nullptr;
+        if (mode != 0) {
+            absorb = expf(-tb.damp[col0 + q] * (float)ctl->h);
+            bnew = cur ? f.B0 : f.B1;
+        }
+        for (int w = row0; w < n; w += gl::T / gl::LINES) {
+            const size_t e = (size_t)w * plane + col0 + q;
+            const float2 g = in[e];
+            if (bnew != nullptr) bnew[e] = cscale(g, tb.filt_w[w] * absorb);
+            lines[q * gl::LP + pidx(w)] = g;
+        }
     }
     __syncthreads();
-    float2 *line = lines + wv * gl::LP;
-    gfft_sel<SPEC, -1>(line, lane, gp, tb.tw_t);
-    const int t0 = lane * gp.c;
-    const int cnt = n - t0 < 0 ? 0 : (n - t0 > gp.c ? gp.c : n - t0);
-    const float sc = 1.0f / (float)n;
-    float2 xv[gl::CM];
-    float imax = 0.0f;
-#pragma unroll
-    for (int k = 0; k < gl::CM; ++k) {
-        xv[k] = make_float2(0.0f, 0.0f);
-        if (k < cnt) {
-            xv[k] = cscale(line[pidx(t0 + k)], sc);
-            imax = fmaxf(imax, ph.a_scale2 * cabs2(xv[k]));
-        }
-    }
-    imax = wave_max(imax);
-    const bool plasma = imax >= ctl->plasma_thr;
-    const bool two = plasma && ph.use_shock;
-    const size_t mine = col0 + wv;   // this wave's line: element (w, line) at w * plane + mine
-    if (!two) {
-        auto total = [&](int k, float c1, float c2) {
-            const float I = ph.a_scale2 * cabs2(xv[k]);
-            const float im = I * (ph.g_n2 + ph.g_n4 * I) - c2;
-            xv[k] = make_float2(-c1 * xv[k].x - im * xv[k].y, -c1 * xv[k].y + im * xv[k].x);
-        };
-        if (plasma) {
-            plasma_walk_g(xv, cnt, lane, ph, tb.lut, total);
-        } else {
-#pragma unroll
-            for (int k = 0; k < gl::CM; ++k)
-                if (k < cnt) total(k, 0.0f, 0.0f);
-        }
-        wave_lds_sync();
-#pragma unroll
-        for (int k = 0; k < gl::CM; ++k)
-            if (k < cnt) line[pidx(t0 + k)] = xv[k];
-    } else {
-        // plasma part P (time) -> LDS, transformed first; Kerr part K (time) -> f.X.  Each lane
-        // reads back only the elements it wrote (its time chunk in f.X, its omega stride in out).
-        auto split = [&](int k, float c1, float c2) {
-            line[pidx(t0 + k)] = make_float2(-c1 * xv[k].x + c2 * xv[k].y,
-                                             -c1 * xv[k].y - c2 * xv[k].x);
-        };
-        wave_lds_sync();
-        plasma_walk_g(xv, cnt, lane, ph, tb.lut, split);
+    gfft_sel<SPEC, -1>(lines + (threadIdx.x >> 6) * gl::LP, threadIdx.x & 63, gp, tb.tw_t);
+    bool two;
+    {
+        const int tid = opaque_i(threadIdx.x), wv = tid >> 6, lane = tid & 63;
+        float2 *line = lines + wv * gl::LP;
+        const int t0 = lane * gp.c;
+        const int cnt = n - t0 < 0 ? 0 : (n - t0 > gp.c ? gp.c : n - t0);
+        const float sc = 1.0f / (float)n;
+        float2 xv[gl::CM];
+        float imax = 0.0f;
 #pragma unroll
         for (int k = 0; k < gl::CM; ++k) {
+            xv[k] = make_float2(0.0f, 0.0f);
             if (k < cnt) {
+                xv[k] = cscale(line[pidx(t0 + k)], sc);
+                imax = fmaxf(imax, ph.a_scale2 * cabs2(xv[k]));
+            }
+        }
+        imax = wave_max(imax);
+        const bool plasma = imax >= ctl->plasma_thr;
+        two = plasma && ph.use_shock;
+        if (!two) {
+            auto total = [&](int k, float c1, float c2) {
                 const float I = ph.a_scale2 * cabs2(xv[k]);
-                const float kap = I * (ph.g_n2 + ph.g_n4 * I);
-                f.X[(size_t)(t0 + k) * plane + mine] = make_float2(-kap * xv[k].y, kap * xv[k].x);
+                const float im = I * (ph.g_n2 + ph.g_n4 * I) - c2;
+                xv[k] = make_float2(-c1 * xv[k].x - im * xv[k].y, -c1 * xv[k].y + im * xv[k].x);
+            };
+            if (plasma) {
+                plasma_walk_g(xv, cnt, lane, *phd, tb.lut, total);
+            } else {
+#pragma unroll
+                for (int k = 0; k < gl::CM; ++k)
+                    if (k < cnt) total(k, 0.0f, 0.0f);
+            }
+            wave_lds_sync();
+#pragma unroll
+            for (int k = 0; k < gl::CM; ++k)
+                if (k < cnt) line[pidx(t0 + k)] = xv[k];
+        } else {
+            // plasma part P (time) -> LDS, transformed first; Kerr part K (time) -> f.X.  Each
+            // lane reads back only the elements it wrote (its time chunk in f.X, its omega
+            // stride in out).  This wave's line is element (w, line) at w * plane + mine.
+            const size_t mine = col0_of() + wv;
+            auto split = [&](int k, float c1, float c2) {
+                line[pidx(t0 + k)] = make_float2(-c1 * xv[k].x + c2 * xv[k].y,
+                                                 -c1 * xv[k].y - c2 * xv[k].x);
+            };
+            wave_lds_sync();
+            plasma_walk_g(xv, cnt, lane, *phd, tb.lut, split);
+#pragma unroll
+            for (int k = 0; k < gl::CM; ++k) {
+                if (k < cnt) {
+                    const float I = ph.a_scale2 * cabs2(xv[k]);
+                    const float kap = I * (ph.g_n2 + ph.g_n4 * I);
+                    f.X[(size_t)(t0 + k) * plane + mine] =
+                        make_float2(-kap * xv[k].y, kap * xv[k].x);
+                }
             }
         }
     }
     // forward transforms: one (Kerr [+ plasma]), or two (plasma, then Kerr)
     for (int pass = 0;; ++pass) {
         wave_lds_sync();
-        gfft_sel<SPEC, 1>(line, lane, gp, tb.tw_t);
+        gfft_sel<SPEC, 1>(lines + (threadIdx.x >> 6) * gl::LP, threadIdx.x & 63, gp, tb.tw_t);
+        const int tid = opaque_i(threadIdx.x), wv = tid >> 6, lane = tid & 63;
+        float2 *line = lines + wv * gl::LP;
         if (!two) {
             const bool sh = ph.use_shock != 0;
             for (int w = lane; w < n; w += 64) {
@@ -3448,7 +3464,10 @@ __global__ __launch_bounds__(gl::T) void gtpass_kernel(Fields3D f, const float2 
             }
             break;
         }
+        const size_t mine = col0_of() + wv;
         if (pass == 0) {
+            const int t0 = lane * gp.c;
+            const int cnt = n - t0 < 0 ? 0 : (n - t0 > gp.c ? gp.c : n - t0);
             for (int w = lane; w < n; w += 64)
                 out[(size_t)w * plane + mine] = cscale(line[pidx(w)], tb.mask_w[w]);
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
@@ -3464,11 +3483,16 @@ __global__ __launch_bounds__(gl::T) void gtpass_kernel(Fields3D f, const float2 
         break;
     }
     __syncthreads();
-    for (int w = row0; w < n; w += gl::T / gl::LINES) {
-        const size_t e = (size_t)w * plane + col0 + q;
-        float2 val = lines[q * gl::LP + pidx(w)];
-        if (mode == 1) val = csub(f.Y4[e], val);
-        out[e] = val;
+    {
+        const int tid = opaque_i(threadIdx.x);
+        const int q = tid % gl::LINES, row0 = tid / gl::LINES;
+        const size_t col0 = col0_of();
+        for (int w = row0; w < n; w += gl::T / gl::LINES) {
+            const size_t e = (size_t)w * plane + col0 + q;
+            float2 val = lines[q * gl::LP + pidx(w)];
+            if (mode == 1) val = csub(f.Y4[e], val);
+            out[e] = val;
+        }
     }
 }
 
@@ -3589,6 +3613,7 @@ struct uppe3d_plan {
     Fields3D f;
     Tabs3D tb;
     Phys3D ph;
+    const Phys3D *phd;   // device copy of ph (functions taking it by reference read it there)
     Ctl3D *ctl;
     int *list;    // t-pass workgroups deferred to the plasma instantiation
     int ncu;      // compute units of the device (grid of the plasma launch)
@@ -3615,7 +3640,7 @@ size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 struct Layout3D {
     size_t tw_t, tw_x, tw_y, twA, twB, twC, tw128, tw256, d64, kref64, ph_d, ph_r, d_w, k_w, k2_w, kref_w, valid_w, filt_w, shock_w, mask_w, kx2, ky2,
-        damp, lut, ctl, list, fields[7], total;
+        damp, lut, ctl, list, phys, fields[7], total;
 };
 
 Layout3D layout3d(const uppe3d_desc &d) {
@@ -3652,6 +3677,7 @@ Layout3D layout3d(const uppe3d_desc &d) {
     L.lut = take(8 * (size_t)d.lut_n);
     L.ctl = take(sizeof(Ctl3D));
     L.list = take(sizeof(int) * (size_t)d.nx * d.ny);
+    L.phys = take(sizeof(Phys3D));
     const size_t field = sizeof(float2) * (size_t)d.nx * d.ny * d.nt;
     for (int i = 0; i < 7; ++i) L.fields[i] = take(field);
     L.total = o;
@@ -3740,10 +3766,10 @@ int launch_tpass(uppe3d_plan *p, int mode, int slot, const float2 *in, int in_is
     if (p->generic_t) {
         if (p->d.nt == 2046)
             gtpass_kernel<2046><<<nx * (ny / gl::LINES), gl::T, 0, s>>>(
-                p->f, in, in_is_state, out, p->tb, p->ph, p->ctl, nx, ny, p->gp, mode);
+                p->f, in, in_is_state, out, p->tb, p->ph, p->phd, p->ctl, nx, ny, p->gp, mode);
         else
             gtpass_kernel<0><<<nx * (ny / gl::LINES), gl::T, 0, s>>>(
-                p->f, in, in_is_state, out, p->tb, p->ph, p->ctl, nx, ny, p->gp, mode);
+                p->f, in, in_is_state, out, p->tb, p->ph, p->phd, p->ctl, nx, ny, p->gp, mode);
         UPPE_HIP(hipGetLastError());
         return UPPE_OK;
     }
@@ -4270,6 +4296,8 @@ int uppe3d_plan_create(const uppe3d_desc *d, void *workspace, size_t workspace_b
         if (!(lut_w[j + 1] >= lut_w[j]) || !(lut_w[j + 1] / I1 >= lut_w[j] / I0 * (1.0 - 1e-12)))
             ph.lut_bound_ok = 0;
     }
+    UPPE_HIP(hipMemcpy(ws + L.phys, &ph, sizeof ph, hipMemcpyHostToDevice));
+    p->phd = (const Phys3D *)(ws + L.phys);
     (void)s;
     *out = p;
     return UPPE_OK;
