@@ -1913,12 +1913,21 @@ __global__ __launch_bounds__(xy_threads(NX, NY)) void xyf_kernel(Fields3D f, Tab
 // (kx = 2 (lane & 7) + h + 16 k2, ky = 8 w + (lane >> 3)) is also the layout of the k-space
 // fields V and Q in HBM: value n of thread tid at n * 1024 + tid.
 // Lane twiddles: tw128[k * 64 + lane] = W128^((lane & 7) k).
+// LDS layouts, chosen so that every transpose access of a wavefront is free of bank conflicts
+// when 16 lanes of 64-bit accesses are served per cycle (two-way at most with 32 lanes), with all
+// offsets of an access sequence constant from one per-lane base:
+//   wave region: element (k, l) at 64 k + (k >> 1) + l   (k < 16, l < 64)
+//   plane:       element (row, col) at 133 row + 3 (row & 1) + col   (row, col < 128)
 namespace x128 {
 constexpr int T = 1024;
-constexpr int PK = 65;                 // wave transpose: pitch per k1 (float2)
-constexpr int WREG = 16 * PK;          // float2 per wavefront
-constexpr int PP = 129;                // plane transpose: pitch per ky (float2)
-constexpr int LDS = 16 * WREG > 128 * PP ? 16 * WREG : 128 * PP;
+constexpr int WREG = 64 * 15 + 7 + 64 + 1;   // float2 per wavefront (wave region, rounded even)
+constexpr int PP = 133;                       // plane row pitch (float2)
+constexpr int PLANE = PP * 127 + 3 + 128;     // float2 of the plane
+constexpr int LDS = 16 * WREG > PLANE ? 16 * WREG : PLANE;
+__device__ __forceinline__ constexpr int widx(int k, int l) { return 64 * k + (k >> 1) + l; }
+__device__ __forceinline__ constexpr int pidx(int row, int col) {
+    return PP * row + 3 * (row & 1) + col;
+}
 }  // namespace x128
 
 // Lane twiddles applied in two groups of loads (bounded registers in flight).
@@ -1939,14 +1948,14 @@ __device__ __forceinline__ void fwd128(float2 (&v)[16], float2 *wb, int lane,
     Dft<16, 1>::run(v);
     twiddle128<1>(v, lane, tw0);
 #pragma unroll
-    for (int k = 0; k < 16; ++k) wb[k * x128::PK + lane] = v[k];
+    for (int k = 0; k < 16; ++k) wb[x128::widx(k, lane)] = v[k];
     wave_lds_sync();
     const int sq = lane >> 3, jj = lane & 7;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         float2 u[8];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) u[j] = wb[(2 * jj + h) * x128::PK + 8 * sq + j];
+        for (int j = 0; j < 8; ++j) u[j] = wb[x128::widx(2 * jj + h, 8 * sq + j)];
         Dft<8, 1>::run(u);
 #pragma unroll
         for (int k2 = 0; k2 < 8; ++k2) v[8 * h + k2] = u[k2];
@@ -1964,11 +1973,11 @@ __device__ __forceinline__ void inv128(float2 (&v)[16], float2 *wb, int lane,
         for (int k2 = 0; k2 < 8; ++k2) u[k2] = v[8 * h + k2];
         Dft<8, -1>::run(u);
 #pragma unroll
-        for (int j = 0; j < 8; ++j) wb[(2 * jj + h) * x128::PK + 8 * sq + j] = u[j];
+        for (int j = 0; j < 8; ++j) wb[x128::widx(2 * jj + h, 8 * sq + j)] = u[j];
     }
     wave_lds_sync();
 #pragma unroll
-    for (int k = 0; k < 16; ++k) v[k] = wb[k * x128::PK + lane];
+    for (int k = 0; k < 16; ++k) v[k] = wb[x128::widx(k, lane)];
     wave_lds_sync();
     twiddle128<-1>(v, lane, tw0);
     Dft<16, -1>::run(v);
@@ -1979,13 +1988,15 @@ __device__ __forceinline__ void inv128(float2 (&v)[16], float2 *wb, int lane,
 __device__ __forceinline__ void rows_to_cols(float2 (&v)[16], float2 *P, int wv, int lane) {
     const int sq = lane >> 3, jj = lane & 7;
     __syncthreads();   // wave transpose regions overlap the plane
-    const int wr = opaque_i(2 * jj * x128::PP + 8 * wv + sq);
+    // rows 2 jj + h + 16 k2 (parity h), columns 8 wv + sq; then rows 8 wv + sq, columns jj + 8 r
+    const int wr = opaque_i(x128::pidx(2 * jj, 8 * wv + sq));
 #pragma unroll
     for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int k2 = 0; k2 < 8; ++k2) P[wr + (h + 16 * k2) * x128::PP] = v[8 * h + k2];
+        for (int k2 = 0; k2 < 8; ++k2)
+            P[wr + (h + 16 * k2) * x128::PP + 3 * h] = v[8 * h + k2];
     __syncthreads();
-    const int rd = opaque_i((8 * wv + sq) * x128::PP + jj);
+    const int rd = opaque_i(x128::pidx(8 * wv + sq, jj));
 #pragma unroll
     for (int r = 0; r < 16; ++r) v[r] = P[rd + 8 * r];
     __syncthreads();
@@ -1994,15 +2005,16 @@ __device__ __forceinline__ void rows_to_cols(float2 (&v)[16], float2 *P, int wv,
 __device__ __forceinline__ void cols_to_rows(float2 (&v)[16], float2 *P, int wv, int lane) {
     const int sq = lane >> 3, jj = lane & 7;
     __syncthreads();
-    const int wr = opaque_i((8 * wv + sq) * x128::PP + jj);
+    const int wr = opaque_i(x128::pidx(8 * wv + sq, jj));
 #pragma unroll
     for (int r = 0; r < 16; ++r) P[wr + 8 * r] = v[r];
     __syncthreads();
-    const int rd = opaque_i(2 * jj * x128::PP + 8 * wv + sq);
+    const int rd = opaque_i(x128::pidx(2 * jj, 8 * wv + sq));
 #pragma unroll
     for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int k2 = 0; k2 < 8; ++k2) v[8 * h + k2] = P[rd + (h + 16 * k2) * x128::PP];
+        for (int k2 = 0; k2 < 8; ++k2)
+            v[8 * h + k2] = P[rd + (h + 16 * k2) * x128::PP + 3 * h];
     __syncthreads();
 }
 

@@ -191,7 +191,18 @@ def test_r256_row_algebra():
 
 
 # -- 128 x 128 plane transform: 16 x 8 register DFTs per 8 lanes, wave and plane transposes --
-PK, WREG, PP = 65, 16 * 65, 129
+# LDS layouts of the device code: wave region (k, l) at 64 k + (k >> 1) + l, plane (row, col) at
+# 133 row + 3 (row & 1) + col
+WREG, PP = 1032, 133
+PLANE = PP * 127 + 3 + 128
+
+
+def widx(k, l):
+    return 64 * k + (k >> 1) + l
+
+
+def pidx(row, col):
+    return PP * row + 3 * (row & 1) + col
 
 
 def tw128():
@@ -206,13 +217,13 @@ def fwd128(v, tw):
     lane = np.arange(64)
     v = np.fft.fft(v, axis=1) * tw.T
     wb = np.full(WREG, np.nan + 0j)
-    addr = np.arange(16)[None, :] * PK + lane[:, None]
+    addr = widx(np.arange(16)[None, :], lane[:, None])
     assert len(np.unique(addr)) == addr.size and addr.max() < WREG
     wb[addr] = v
     sq, jj = lane >> 3, lane & 7
     out = np.zeros((64, 16), complex)
     for h in range(2):
-        u = wb[(2 * jj + h)[:, None] * PK + 8 * sq[:, None] + np.arange(8)[None, :]]
+        u = wb[widx((2 * jj + h)[:, None], 8 * sq[:, None] + np.arange(8)[None, :])]
         out[:, 8 * h:8 * h + 8] = np.fft.fft(u, axis=1)
     assert np.isfinite(out).all()
     return out
@@ -224,8 +235,8 @@ def inv128(v, tw):
     wb = np.full(WREG, np.nan + 0j)
     for h in range(2):
         u = np.fft.ifft(v[:, 8 * h:8 * h + 8], axis=1) * 8
-        wb[(2 * jj + h)[:, None] * PK + 8 * sq[:, None] + np.arange(8)[None, :]] = u
-    v = wb[np.arange(16)[None, :] * PK + lane[:, None]]
+        wb[widx((2 * jj + h)[:, None], 8 * sq[:, None] + np.arange(8)[None, :])] = u
+    v = wb[widx(np.arange(16)[None, :], lane[:, None])]
     v = np.fft.ifft(v * np.conj(tw.T), axis=1) * 16
     assert np.isfinite(v).all()
     return v
@@ -237,7 +248,7 @@ def test_x128_plane_algebra():
     A = rng.standard_normal((128, 128)) + 1j * rng.standard_normal((128, 128))
     lane = np.arange(64)
     sq, jj = lane >> 3, lane & 7
-    P = np.full(128 * PP, np.nan + 0j)
+    P = np.full(PLANE, np.nan + 0j)
     regs = []
     for wv in range(16):
         row = 8 * wv + sq
@@ -245,12 +256,12 @@ def test_x128_plane_algebra():
         v = fwd128(v, tw)
         for h in range(2):
             for k2 in range(8):
-                addr = (2 * jj + h + 16 * k2) * PP + 8 * wv + sq
+                addr = pidx(2 * jj + h + 16 * k2, 8 * wv + sq)
                 P[addr] = v[:, 8 * h + k2]
-    assert np.isfinite(P[[ky * PP + x for ky in range(128) for x in range(128)]]).all()
+    assert np.isfinite(P[[pidx(ky, x) for ky in range(128) for x in range(128)]]).all()
     K = np.zeros((128, 128), complex)          # K[kx, ky]
     for wv in range(16):
-        v = P[(8 * wv + sq)[:, None] * PP + jj[:, None] + 8 * np.arange(16)[None, :]]
+        v = P[pidx((8 * wv + sq)[:, None], jj[:, None] + 8 * np.arange(16)[None, :])]
         v = fwd128(v, tw)
         regs.append(v)
         ky = 8 * wv + sq
@@ -259,20 +270,49 @@ def test_x128_plane_algebra():
             K[kx, ky] = v[:, n]
     np.testing.assert_allclose(K, np.fft.fft2(A), atol=1e-9)
     # inverse back to rows
-    P = np.full(128 * PP, np.nan + 0j)
+    P = np.full(PLANE, np.nan + 0j)
     for wv in range(16):
         v = inv128(regs[wv], tw)
-        P[(8 * wv + sq)[:, None] * PP + jj[:, None] + 8 * np.arange(16)[None, :]] = v
+        P[pidx((8 * wv + sq)[:, None], jj[:, None] + 8 * np.arange(16)[None, :])] = v
     out = np.zeros((128, 128), complex)
     for wv in range(16):
         v = np.zeros((64, 16), complex)
         for h in range(2):
             for k2 in range(8):
-                v[:, 8 * h + k2] = P[(2 * jj + h + 16 * k2) * PP + 8 * wv + sq]
+                v[:, 8 * h + k2] = P[pidx(2 * jj + h + 16 * k2, 8 * wv + sq)]
         v = inv128(v, tw) / 16384
         out[(8 * wv + sq)[:, None], jj[:, None] + 8 * np.arange(16)[None, :]] = v
     np.testing.assert_allclose(out, A, atol=1e-12)
-    assert max(16 * WREG, 128 * PP) * 8 <= 160 * 1024
+    assert max(16 * WREG, PLANE) * 8 <= 160 * 1024
+
+
+def bank_degree(addr, group):
+    """Largest number of lanes of one group of `group` consecutive lanes whose 64-bit accesses
+    fall on the same pair of 4-byte banks (2 group banks)."""
+    addr = np.asarray(addr)
+    return max(np.bincount(addr[g:g + group] % group, minlength=group).max()
+               for g in range(0, 64, group))
+
+
+def test_x128_transposes_conflict_free():
+    """Every wavefront access of the four transposes is conflict-free with 16 lanes of 64-bit
+    accesses per cycle and at most two-way with 32 (the previous layout, pitch 65 and 129, was
+    two-way and up to four-way); offsets from the per-lane base are the same for every lane."""
+    lane = np.arange(64)
+    sq, jj = lane >> 3, lane & 7
+    pats = []
+    pats += [widx(k, lane) for k in range(16)]
+    pats += [widx(2 * jj + h, 8 * sq + j) for h in range(2) for j in range(8)]
+    pats += [pidx(2 * jj + h + 16 * k2, 8 * wv + sq) for h in range(2) for k2 in range(8)
+             for wv in range(16)]
+    pats += [pidx(8 * wv + sq, jj + 8 * r) for r in range(16) for wv in range(16)]
+    assert max(bank_degree(a, 16) for a in pats) == 1
+    assert max(bank_degree(a, 32) for a in pats) <= 2
+    # constant offsets: the device adds (h + 16 k2) PP + 3 h to one per-lane base
+    for h in range(2):
+        for k2 in range(8):
+            np.testing.assert_array_equal(pidx(2 * jj + h + 16 * k2, 8 * sq),
+                                          pidx(2 * jj, 8 * sq) + (h + 16 * k2) * PP + 3 * h)
 
 
 def cis_f32(x):
