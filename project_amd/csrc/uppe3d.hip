@@ -1392,7 +1392,7 @@ __global__ __launch_bounds__(64 * LINES) void load_kernel(const float2 *__restri
                                                           Fields3D f, Tabs3D tb, Ctl3D *ctl,
                                                           float inv_scale, int nx, int ny) {
     constexpr int EPT = NT / 64;
-    constexpr int LDT = NT + NT / 16;
+    constexpr int LDT = NT + NT / 16 + 1;   // odd: staging rows of LINES lines on distinct banks
     constexpr int T = 64 * LINES;
     __shared__ float2 lds[LINES * LDT];
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
@@ -1420,7 +1420,7 @@ __global__ __launch_bounds__(64 * LINES) void store_kernel(float2 *__restrict__ 
                                                            Tabs3D tb, Ctl3D *ctl, float scale,
                                                            int nx, int ny) {
     constexpr int EPT = NT / 64;
-    constexpr int LDT = NT + NT / 16;
+    constexpr int LDT = NT + NT / 16 + 1;   // odd: staging rows of LINES lines on distinct banks
     constexpr int T = 64 * LINES;
     __shared__ float2 lds[LINES * LDT];
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
@@ -2649,7 +2649,8 @@ __global__ __launch_bounds__(1024) void colpass_kernel(const float2 *src, float2
                                                        float scale, const float2 *tw, Fields3D f,
                                                        const Ctl3D *ctl, int src_state,
                                                        int check_done) {
-    constexpr int EPT = NX / 64, LP = NX + NX / 16;
+    // line pitch odd: the 16 columns q of one staging access fall on distinct bank pairs
+    constexpr int EPT = NX / 64, LP = NX + NX / 16 + 1;
     __shared__ float2 lines[16 * LP];
     if (check_done && ctl->done) return;
     if (src_state) src = ctl->cur ? f.B1 : f.B0;
@@ -3003,7 +3004,9 @@ constexpr int NTMAX = 2048;
 constexpr int LINES = 8;                    // lines (wavefronts) per t-pass workgroup
 constexpr int T = 64 * LINES;
 constexpr int CM = NTMAX / 64;              // samples per lane, at most
-constexpr int LP = NTMAX + NTMAX / 16;      // line pitch (float2): pidx(NTMAX - 1) < LP
+// line pitch (float2): pidx(NTMAX - 1) < LP, and LP = 2 mod 16, so that the HBM staging
+// accesses (8 lines q at one row w, two rows per 16 lanes) fall on distinct bank pairs
+constexpr int LP = NTMAX + NTMAX / 16 + 2;
 constexpr int MAXST = 12;                   // stages (2^11 needs 11 radix-2 stages at most)
 constexpr double PI = 3.141592653589793238462643383279502884;
 
