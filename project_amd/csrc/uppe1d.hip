@@ -69,6 +69,14 @@ __device__ __forceinline__ const float2 *opaque_tw(const float2 *p) {
     return p;
 }
 
+// Intermediate stages keep their data at fpad(n) = n + n / 32: the outputs of a butterfly
+// are NS apart, so without padding 16 lanes writing at stride R NS (8 at NS = 1) shared one
+// bank pair (8-way conflicts); padded, two-way at most.  The first stage reads and the last
+// stage writes natural order, so callers see unpadded buffers of N points; the buffers hold
+// N + N / 32 points (lds_pad).
+__device__ __forceinline__ constexpr int fpad(int n) { return n + (n >> 5); }
+constexpr int lds_pad(int n) { return n + n / 32; }
+
 template <int T, int N, int NS, int DIR>
 struct LdsFFT {
     __device__ __forceinline__ static float2 *run(float2 *src, float2 *dst,
@@ -78,13 +86,15 @@ struct LdsFFT {
         constexpr int R = REM >= 8 ? 8 : (REM == 4 ? 4 : 2);
         constexpr int NB = N / R;
         constexpr int STEP = N / (NS * R);   // twiddle stride per unit of k
+        constexpr bool PIN = NS > 1;         // input padded (not the first stage)
+        constexpr bool POUT = NS * R < N;    // output padded (not the last stage)
 #pragma unroll
         for (int j0 = 0; j0 < NB; j0 += T) {
             const int j = j0 + tid;
             if (NB % T == 0 || j < NB) {
                 float2 v[R];
 #pragma unroll
-                for (int r = 0; r < R; ++r) v[r] = src[j + r * NB];
+                for (int r = 0; r < R; ++r) v[r] = src[PIN ? fpad(j + r * NB) : j + r * NB];
                 const int k = j & (NS - 1);
                 if constexpr (NS > 1) {
 #pragma unroll
@@ -96,7 +106,7 @@ struct LdsFFT {
                 Dft<R, DIR>::run(v);
                 const int b = (j - k) * R + k;   // (j / NS) NS R + j % NS
 #pragma unroll
-                for (int r = 0; r < R; ++r) dst[b + r * NS] = v[r];
+                for (int r = 0; r < R; ++r) dst[POUT ? fpad(b + r * NS) : b + r * NS] = v[r];
             }
         }
         __syncthreads();
@@ -611,7 +621,7 @@ __global__ void zero_counter(int *c) { *c = 0; }
 template <int T, int NMAX>
 __global__ __launch_bounds__(T) void uppe1d_kernel(const Inst1D *__restrict__ insts, int count,
                                                    int *counter) {
-    __shared__ float2 bufA[NMAX], bufB[NMAX];
+    __shared__ float2 bufA[lds_pad(NMAX)], bufB[lds_pad(NMAX)];
     __shared__ double scan[64];
     __shared__ double bc[8];
     __shared__ int next;
@@ -817,7 +827,7 @@ __device__ void run_chain(const Chain1D &C, float2 *bufA, float2 *bufB, double *
 
 template <int T, int NMAX>
 __global__ __launch_bounds__(T) void uppe1d_chain_kernel(const Chain1D *__restrict__ chain) {
-    __shared__ float2 bufA[NMAX], bufB[NMAX];
+    __shared__ float2 bufA[lds_pad(NMAX)], bufB[lds_pad(NMAX)];
     __shared__ double scan[64];
     const Chain1D C = *chain;
     const int nt = C.P.nt, tid = threadIdx.x;
@@ -1263,7 +1273,7 @@ __device__ __forceinline__ void adjoint_e(const Adj1D &J, float2 *bufA, float2 *
 template <int T, int NMAX>
 __global__ __launch_bounds__(T) void uppe1d_adjoint_kernel(const Adj1D *__restrict__ jobs,
                                                            int count) {
-    __shared__ float2 bufA[NMAX], bufB[NMAX];
+    __shared__ float2 bufA[lds_pad(NMAX)], bufB[lds_pad(NMAX)];
     __shared__ double scan[64];
     for (int k = blockIdx.x; k < count; k += gridDim.x) {
         const Adj1D J = jobs[k];
