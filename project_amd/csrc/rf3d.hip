@@ -48,18 +48,30 @@ constexpr size_t kAlign = 256;
 // W_N^(k r N/(R ns)) with k = j % ns (W_N^m = exp(-2 pi i m / N) in tw; DIR > 0 forward,
 // DIR < 0 inverse), an R-point DFT, output y[(j / ns) R ns + k + r ns].  Every stage ends
 // with a workgroup barrier.  Returns the buffer (a or b) that holds the result.
+// Between stages the data sit at gpad(n) = n + n / 16 (the first stage reads and the last
+// writes natural order): the radix-4 outputs of consecutive butterflies are 4 apart, which
+// without padding put 16 lanes on 4 bank pairs; the buffers hold N + N / 16 points per line.
+__device__ __forceinline__ constexpr int gpad(int n) { return n + (n >> 4); }
+constexpr int gpad_len(int n) { return n + n / 16; }
+
 template <int N, int DIR>
 __device__ float2 *group_fft(float2 *a, float2 *b, int LD, int g, int lid, int per,
                              const float2 *__restrict__ tw) {
+    constexpr int L4 = __builtin_ctz(N) / 2;                 // radix-4 stages
+    constexpr int NSTAGE = L4 + (__builtin_ctz(N) & 1);     // + one radix-2 stage if odd
     float2 *src = a, *dst = b;
-    int ns = 1;
-    for (; ns * 4 <= N; ns <<= 2) {
+    int ns = 1, si = 0;
+    for (; ns * 4 <= N; ns <<= 2, ++si) {
         float2 *s = src + g * LD, *d = dst + g * LD;
+        const bool pin = si > 0, pout = si + 1 < NSTAGE;
         for (int j = lid; j < N / 4; j += per) {
             const int k = j & (ns - 1);
             float2 v[4];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) v[r] = s[j + r * (N / 4)];
+            for (int r = 0; r < 4; ++r) {
+                const int n = j + r * (N / 4);
+                v[r] = s[pin ? gpad(n) : n];
+            }
             if (ns > 1) {
                 const int st = k * (N / (4 * ns));
 #pragma unroll
@@ -71,19 +83,21 @@ __device__ float2 *group_fft(float2 *a, float2 *b, int LD, int g, int lid, int p
             Dft<4, DIR>::run(v);
             const int o = (j - k) * 4 + k;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) d[o + r * ns] = v[r];
+            for (int r = 0; r < 4; ++r) d[pout ? gpad(o + r * ns) : o + r * ns] = v[r];
         }
         __syncthreads();
         float2 *t = src;
         src = dst;
         dst = t;
     }
-    if (ns < N) {   // N = 2 ns
+    if (ns < N) {   // N = 2 ns, the last stage (its input padded when it is not the first)
         float2 *s = src + g * LD, *d = dst + g * LD;
+        const bool pin = si > 0;
         for (int j = lid; j < N / 2; j += per) {
-            const float2 u0 = s[j];
+            const float2 u0 = s[pin ? gpad(j) : j];
             const float2 w = tw[j];
-            const float2 u1 = DIR > 0 ? cmul(s[j + N / 2], w) : cmulc(s[j + N / 2], w);
+            const float2 x1 = s[pin ? gpad(j + N / 2) : j + N / 2];
+            const float2 u1 = DIR > 0 ? cmul(x1, w) : cmulc(x1, w);
             d[j] = cadd(u0, u1);
             d[j + ns] = csub(u0, u1);
         }
@@ -98,7 +112,7 @@ __device__ float2 *group_fft(float2 *a, float2 *b, int LD, int g, int lid, int p
 template <int NT>
 __global__ __launch_bounds__(LT) void rfft_kernel(const float *__restrict__ in, float2 *out,
                                                   float scale, const float2 *__restrict__ tw) {
-    __shared__ float2 a[NT], b[NT];
+    __shared__ float2 a[gpad_len(NT)], b[gpad_len(NT)];
     constexpr int W = NT / 2 + 1;
     const int tid = threadIdx.x;
     const float *src = in + (size_t)blockIdx.x * NT;
@@ -117,7 +131,7 @@ __global__ __launch_bounds__(LT) void irfft_kernel(const float2 *__restrict__ in
                                                    const float *__restrict__ filt,
                                                    const float *__restrict__ damp, float h,
                                                    float scale, const float2 *__restrict__ tw) {
-    __shared__ float2 a[NT], b[NT];
+    __shared__ float2 a[gpad_len(NT)], b[gpad_len(NT)];
     constexpr int W = NT / 2 + 1;
     const int tid = threadIdx.x;
     const float2 *src = in + (size_t)blockIdx.x * W;
@@ -206,7 +220,7 @@ __global__ __launch_bounds__(LT) void source_kernel(const float2 *__restrict__ P
                                                     const float *__restrict__ J, float2 *out,
                                                     Ops ops, const float2 *__restrict__ tw) {
     constexpr int W = NT / 2 + 1;
-    __shared__ float2 a[NT], b[NT];
+    __shared__ float2 a[gpad_len(NT)], b[gpad_len(NT)];
     const int tid = threadIdx.x;
     const size_t lt = (size_t)blockIdx.x * NT, lw = (size_t)blockIdx.x * W;
     for (int t = tid; t < NT; t += LT) a[t] = make_float2(J[lt + t], 0.0f);
@@ -228,7 +242,7 @@ template <int N, int DIR>
 __global__ __launch_bounds__(LT) void xy_kernel(const float2 *src, float2 *dst, int W,
                                                 int stride, int ostride, int ntw,
                                                 const float2 *__restrict__ tw) {
-    constexpr int LD = N + 1, PER = LT / G;
+    constexpr int LD = gpad_len(N) | 1, PER = LT / G;   // odd: the G staged columns on distinct banks
     __shared__ float2 a[G * LD], b[G * LD];
     const int tid = threadIdx.x;
     const int o = blockIdx.x / ntw, w0 = (blockIdx.x % ntw) * G;
@@ -280,7 +294,7 @@ __global__ __launch_bounds__(LT) void field_kernel(const float2 *__restrict__ in
                                                    const float *__restrict__ lw, float scale,
                                                    float in_scale,
                                                    const float2 *__restrict__ tw) {
-    __shared__ float2 a[NT], b[NT];
+    __shared__ float2 a[gpad_len(NT)], b[gpad_len(NT)];
     constexpr int W = NT / 2 + 1;
     const int tid = threadIdx.x;
     const float2 *src = in + (size_t)blockIdx.x * W;
