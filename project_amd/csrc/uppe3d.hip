@@ -1593,15 +1593,20 @@ __device__ __forceinline__ float2 cis_f(float x) {
 // evanescent kz D = -sqrt(kperp^2 - k^2) - i k_ref, and D = 0 where omega0 + omega <= 0.
 // The per-plane factors ed = exp(i s d_w), eref = exp(-i s k_ref) come from the caller (FP64
 // phases, phase_kernel); the per-element phase s kperp^2 / (kz + k) is small.
-__device__ __forceinline__ float2 eprop_k(float kp2, int w, float s, const Tabs3D &tb,
-                                          float2 ed, float2 eref) {
-    if (tb.valid_w[w] == 0.0f) return make_float2(1.0f, 0.0f);
-    const float k2 = tb.k2_w[w];
+// With the plane's table entries already read: vw = valid_w[w], k2 = k2_w[w], kw = k_w[w].
+__device__ __forceinline__ float2 eprop_kp(float kp2, float s, float vw, float k2, float kw,
+                                           float2 ed, float2 eref) {
+    if (vw == 0.0f) return make_float2(1.0f, 0.0f);
     if (kp2 <= k2) {
         const float kz = sqrtf(k2 - kp2);
-        return cmul(ed, cis_f(-s * kp2 * __builtin_amdgcn_rcpf(kz + tb.k_w[w])));
+        return cmul(ed, cis_f(-s * kp2 * __builtin_amdgcn_rcpf(kz + kw)));
     }
     return cscale(eref, expf(-s * sqrtf(kp2 - k2)));
+}
+
+__device__ __forceinline__ float2 eprop_k(float kp2, int w, float s, const Tabs3D &tb,
+                                          float2 ed, float2 eref) {
+    return eprop_kp(kp2, s, tb.valid_w[w], tb.k2_w[w], tb.k_w[w], ed, eref);
 }
 
 // Per-plane factors exp(i s d_w), exp(-i s k_ref) for the current step h (s = h / 2), phases
@@ -1924,16 +1929,29 @@ constexpr int WREG = 64 * 15 + 7 + 64 + 1;   // float2 per wavefront (wave regio
 constexpr int PP = 133;                       // plane row pitch (float2)
 constexpr int PLANE = PP * 127 + 3 + 128;     // float2 of the plane
 constexpr int LDS = 16 * WREG > PLANE ? 16 * WREG : PLANE;
+static_assert(T == 16 * 64, "one lane-twiddle entry (tw128) per thread");
 __device__ __forceinline__ constexpr int widx(int k, int l) { return 64 * k + (k >> 1) + l; }
 __device__ __forceinline__ constexpr int pidx(int row, int col) {
     return PP * row + 3 * (row & 1) + col;
 }
 }  // namespace x128
 
-// Lane twiddles applied in two groups of loads (bounded registers in flight).
-template <int DIR>
+// Lane twiddles applied in two groups of loads (bounded registers in flight).  TL: tw0 is a
+// copy of the table in LDS, so the transform issues no vector-memory load (vector-memory
+// loads retire in order, and a twiddle load behind a plane still in flight would wait for it).
+template <int DIR, bool TL = false>
 __device__ __forceinline__ void twiddle128(float2 (&v)[16], int lane,
                                            const float2 *__restrict__ tw0) {
+    if constexpr (TL) {
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+#pragma unroll
+            for (int k = (g == 0 ? 1 : 8); k < (g == 0 ? 8 : 16); ++k)
+                v[k] = DIR > 0 ? cmul(v[k], tw0[k * 64 + lane]) : cmulc(v[k], tw0[k * 64 + lane]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        return;
+    }
 #pragma unroll
     for (int g = 0; g < 2; ++g) {
         const float2 *tw = opaque(tw0);
@@ -1943,10 +1961,11 @@ __device__ __forceinline__ void twiddle128(float2 (&v)[16], int lane,
     }
 }
 
+template <bool TL = false>
 __device__ __forceinline__ void fwd128(float2 (&v)[16], float2 *wb, int lane,
                                        const float2 *__restrict__ tw0) {
     Dft<16, 1>::run(v);
-    twiddle128<1>(v, lane, tw0);
+    twiddle128<1, TL>(v, lane, tw0);
 #pragma unroll
     for (int k = 0; k < 16; ++k) wb[x128::widx(k, lane)] = v[k];
     wave_lds_sync();
@@ -1963,6 +1982,7 @@ __device__ __forceinline__ void fwd128(float2 (&v)[16], float2 *wb, int lane,
     wave_lds_sync();
 }
 
+template <bool TL = false>
 __device__ __forceinline__ void inv128(float2 (&v)[16], float2 *wb, int lane,
                                        const float2 *__restrict__ tw0) {
     const int sq = lane >> 3, jj = lane & 7;
@@ -1979,7 +1999,7 @@ __device__ __forceinline__ void inv128(float2 (&v)[16], float2 *wb, int lane,
 #pragma unroll
     for (int k = 0; k < 16; ++k) v[k] = wb[x128::widx(k, lane)];
     wave_lds_sync();
-    twiddle128<-1>(v, lane, tw0);
+    twiddle128<-1, TL>(v, lane, tw0);
     Dft<16, -1>::run(v);
 }
 
@@ -2019,30 +2039,28 @@ __device__ __forceinline__ void cols_to_rows(float2 (&v)[16], float2 *P, int wv,
 }
 
 // plane (row-major, HBM) -> FFT2 in the k-space layout
-__device__ __forceinline__ void fft2_128(float2 (&v)[16], const float2 *__restrict__ src,
-                                         float2 *lds, int wv, int lane, const float2 *tw) {
+// the thread's 16 values of a plane (row-major, HBM), in the row-transform input layout
+__device__ __forceinline__ void load128(float2 (&v)[16], const float2 *__restrict__ src, int wv,
+                                        int lane) {
     const int row = 8 * wv + (lane >> 3), j = lane & 7;
 #pragma unroll
     for (int r = 0; r < 16; ++r) v[r] = src[row * 128 + j + 8 * r];
-    fwd128(v, lds + wv * x128::WREG, lane, tw);
-    rows_to_cols(v, lds, wv, lane);
-    fwd128(v, lds + wv * x128::WREG, lane, tw);
 }
 
-// as fft2_128, of src * exp(damp h) (damp per plane element)
-__device__ __forceinline__ void fft2_128_unabsorb(float2 (&v)[16], const float2 *__restrict__ src,
-                                                  const float *__restrict__ damp, float h,
-                                                  float2 *lds, int wv, int lane,
-                                                  const float2 *tw) {
-    const int row = 8 * wv + (lane >> 3), j = lane & 7;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int e = row * 128 + j + 8 * r;
-        v[r] = cscale(src[e], expf(damp[e] * h));
-    }
-    fwd128(v, lds + wv * x128::WREG, lane, tw);
+// FFT2 of a plane already in registers (load128 layout) -> k-space layout
+template <bool TL = false>
+__device__ __forceinline__ void fft2_128_regs(float2 (&v)[16], float2 *lds, int wv, int lane,
+                                              const float2 *tw) {
+    fwd128<TL>(v, lds + wv * x128::WREG, lane, tw);
     rows_to_cols(v, lds, wv, lane);
-    fwd128(v, lds + wv * x128::WREG, lane, tw);
+    fwd128<TL>(v, lds + wv * x128::WREG, lane, tw);
+}
+
+template <bool TL = false>
+__device__ __forceinline__ void fft2_128(float2 (&v)[16], const float2 *__restrict__ src,
+                                         float2 *lds, int wv, int lane, const float2 *tw) {
+    load128(v, src, wv, lane);
+    fft2_128_regs<TL>(v, lds, wv, lane, tw);
 }
 
 // FFT2 of the difference srcA - srcB of two planes (row-major, HBM)
@@ -2058,11 +2076,12 @@ __device__ __forceinline__ void fft2_128_diff(float2 (&v)[16], const float2 *__r
 }
 
 // k-space layout -> iFFT2 / (128 * 128) -> plane (row-major, HBM)
+template <bool TL = false>
 __device__ __forceinline__ void ifft2_128(float2 (&v)[16], float2 *__restrict__ dst,
                                           float2 *lds, int wv, int lane, const float2 *tw) {
-    inv128(v, lds + wv * x128::WREG, lane, tw);
+    inv128<TL>(v, lds + wv * x128::WREG, lane, tw);
     cols_to_rows(v, lds, wv, lane);
-    inv128(v, lds + wv * x128::WREG, lane, tw);
+    inv128<TL>(v, lds + wv * x128::WREG, lane, tw);
     const int row = 8 * wv + (lane >> 3), j = lane & 7;
     constexpr float s = 1.0f / 16384.0f;
 #pragma unroll
@@ -2151,10 +2170,13 @@ __global__ __launch_bounds__(x128::T) void xy128_kernel(Fields3D f, Tabs3D tb, C
 }
 
 // The xy passes of xyf_kernel (MODE 8, 9, 10) for 128 x 128 planes, the plane in registers
-// (16 values per thread) between the transforms.
+// (16 values per thread) between the transforms.  The second input plane is loaded at the
+// start, so its HBM read overlaps the first plane's transform; the lane twiddles are read from
+// a copy in LDS, so nothing in the transform waits on that load.
 template <int MODE>
 __global__ __launch_bounds__(x128::T) void xy128f_kernel(Fields3D f, Tabs3D tb, Ctl3D *ctl) {
     __shared__ float2 lds[x128::LDS];
+    __shared__ float2 twl[16 * 64];
     __shared__ unsigned int red[x128::T / 64];
     __shared__ double reds[x128::T / 64];
     if (ctl->done) return;
@@ -2162,13 +2184,26 @@ __global__ __launch_bounds__(x128::T) void xy128f_kernel(Fields3D f, Tabs3D tb, 
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const size_t pb = (size_t)w * 16384;
     const float h = (float)ctl->h;
+    const bool l2 = ctl->l2 != 0;   // read once: a re-read after the atomics would wait on
+                                    // every vector-memory load in flight
     float2 *const Xp = f.X + pb, *const Yp = f.Yw + pb;
-    float2 a[16];
+    // small tables first: vector-memory loads retire in issue order
+    const float2 twv = tb.tw128[tid];   // 16 * 64 entries, one per thread
+    float2 a[16], b[16];
     if (MODE == 10) {
         const float2 *Bn = (ctl->cur ? f.B0 : f.B1) + pb;
+        const int row = 8 * wv + (lane >> 3), j = lane & 7;
+        float dm[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) dm[r] = tb.damp[row * 128 + j + 8 * r];
+        load128(a, Bn, wv, lane);
+        twl[tid] = twv;
+        __syncthreads();
         const float fi = 1.0f / tb.filt_w[w];
         const float fi2 = fi * fi;
-        fft2_128_unabsorb(a, Bn, tb.damp, h, lds, wv, lane, tb.tw128);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) a[r] = cscale(a[r], expf(dm[r] * h));
+        fft2_128_regs<true>(a, lds, wv, lane, twl);
         unsigned int m = 0;
         float sq = 0.0f;
 #pragma unroll
@@ -2179,63 +2214,61 @@ __global__ __launch_bounds__(x128::T) void xy128f_kernel(Fields3D f, Tabs3D tb, 
             sq += a2;
         }
         block_max_to<x128::T>(m, &ctl->maxA_bits, red, tid);
-        if (ctl->l2) block_sum_to<x128::T>(sq, &ctl->sumA, reds, tid);
-        fft2_128(a, Xp, lds, wv, lane, tb.tw128);   // barriers inside: red is free again
+        if (l2) block_sum_to<x128::T>(sq, &ctl->sumA, reds, tid);
+        fft2_128<true>(b, Xp, lds, wv, lane, twl);   // barriers inside: red is free again
         m = 0;
         sq = 0.0f;
 #pragma unroll
         for (int n = 0; n < 16; ++n) {
-            const float a2 = cabs2(a[n]);
+            const float a2 = cabs2(b[n]);
             const unsigned int u = __float_as_uint(a2);
             m = u > m ? u : m;
             sq += a2;
         }
         block_max_to<x128::T>(m, &ctl->maxD_bits, red, tid);
-        if (ctl->l2) block_sum_to<x128::T>(sq, &ctl->sumD, reds, tid);
+        if (l2) block_sum_to<x128::T>(sq, &ctl->sumD, reds, tid);
         return;
     }
+    __shared__ float kxl[128];
     const float2 ed = tb.ph_d[w], eref = tb.ph_r[w];
+    const float vw = tb.valid_w[w], k2w = tb.k2_w[w], kw = tb.k_w[w];
     const int ky = 8 * wv + (lane >> 3), kxb = 2 * (lane & 7);
+    const float kxv = tb.kx2[tid & 127];
     const float kyy = tb.ky2[ky];
     const float2 *B = (ctl->cur ? f.B1 : f.B0) + pb;
-    float2 b[16];
-    if (MODE == 8) {
-        float2 E[16];
-        fft2_128(a, B, lds, wv, lane, tb.tw128);
+    load128(a, B, wv, lane);
+    __builtin_amdgcn_sched_barrier(0);   // every load of a before any load of b
+    load128(b, MODE == 8 ? Yp : Xp, wv, lane);   // Y1 (MODE 8), h Y3 (MODE 9)
+    twl[tid] = twv;
+    if (tid < 128) kxl[tid] = kxv;
+    __syncthreads();
+    float2 E[16];
+    fft2_128_regs<true>(a, lds, wv, lane, twl);
 #pragma unroll
-        for (int n = 0; n < 16; ++n) {
-            const int kx = kxb + (n >> 3) + 16 * (n & 7);
-            E[n] = eprop_k(tb.kx2[kx] + kyy, w, 0.5f * h, tb, ed, eref);
-            a[n] = cmul(E[n], a[n]);   // E F(B)
-            __builtin_amdgcn_sched_barrier(0);   // one element at a time: bounded registers
-        }
-        fft2_128(b, Yp, lds, wv, lane, tb.tw128);
+    for (int n = 0; n < 16; ++n) {
+        const int kx = kxb + (n >> 3) + 16 * (n & 7);
+        E[n] = eprop_kp(kxl[kx] + kyy, 0.5f * h, vw, k2w, kw, ed, eref);
+        a[n] = cmul(E[n], a[n]);   // E F(B)
+        __builtin_amdgcn_sched_barrier(0);   // one element at a time: bounded registers
+    }
+    fft2_128_regs<true>(b, lds, wv, lane, twl);
+    if (MODE == 8) {
 #pragma unroll
         for (int n = 0; n < 16; ++n) {
             const float2 t = cmul(E[n], b[n]);   // E F(Y1)
             a[n] = caxpy(a[n], 0.5f * h, t);     // X2
             b[n] = cscale(t, h / 6.0f);          // Zh
         }
-        ifft2_128(a, Xp, lds, wv, lane, tb.tw128);
-        ifft2_128(b, Yp, lds, wv, lane, tb.tw128);
+        ifft2_128<true>(a, Xp, lds, wv, lane, twl);
+        ifft2_128<true>(b, Yp, lds, wv, lane, twl);
     } else {
-        float2 E[16];
-        fft2_128(a, B, lds, wv, lane, tb.tw128);
-#pragma unroll
-        for (int n = 0; n < 16; ++n) {
-            const int kx = kxb + (n >> 3) + 16 * (n & 7);
-            E[n] = eprop_k(tb.kx2[kx] + kyy, w, 0.5f * h, tb, ed, eref);
-            a[n] = cmul(E[n], a[n]);   // E F(B)
-            __builtin_amdgcn_sched_barrier(0);
-        }
-        fft2_128(b, Xp, lds, wv, lane, tb.tw128);
 #pragma unroll
         for (int n = 0; n < 16; ++n) b[n] = cmul(E[n], cadd(a[n], b[n]));   // X4
-        ifft2_128(b, Xp, lds, wv, lane, tb.tw128);
-        fft2_128(b, Yp, lds, wv, lane, tb.tw128);
+        ifft2_128<true>(b, Xp, lds, wv, lane, twl);
+        fft2_128<true>(b, Yp, lds, wv, lane, twl);
 #pragma unroll
         for (int n = 0; n < 16; ++n) b[n] = cmul(E[n], cadd(a[n], b[n]));   // R'
-        ifft2_128(b, Yp, lds, wv, lane, tb.tw128);
+        ifft2_128<true>(b, Yp, lds, wv, lane, twl);
     }
 }
 
