@@ -2803,22 +2803,38 @@ __device__ __forceinline__ void row256_group(int blk, Fields3D &f, Adj3D &a, con
     // value r of a lane: y = j + 16 r before the forward transform and after the inverse,
     // k_y = j + 16 r in between
     const float2 *const tw = tb.tw256;
-    auto E_at = [&](int r) {
-        return eprop_k(kxx + tb.ky2[j + 16 * r], w, 0.5f * h, tb, ed, eref);
-    };
+    // The element loops below store into fields the compiler cannot tell apart from the
+    // tables, so every table or field value they need is read before them, in one batch:
+    // otherwise each element would wait for a full memory round trip behind the last stores.
+    constexpr bool USES_E = MODE == 2 || MODE == 4 || MODE == 22 || MODE == 24;
+    const float vw = tb.valid_w[w], k2w = tb.k2_w[w], kw = tb.k_w[w];
+    float kyv[16];
+    if (USES_E) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) kyv[r] = tb.ky2[j + 16 * r];
+    }
+    auto E_at = [&](int r) { return eprop_kp(kxx + kyv[r], 0.5f * h, vw, k2w, kw, ed, eref); };
     float2 b[16], o[16];
     const float2 *in = (MODE == 5) ? f.X : (ADJ ? a.Xb : f.Yw);
     row256_fwd(b, in + rb + j, wb, lane, tw);
+    float2 vv[16], qv[16];
+    if (MODE == 3 || MODE == 4) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            vv[r] = f.V[rb + j + 16 * r];
+            qv[r] = f.Q[rb + j + 16 * r];
+        }
+    }
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const size_t e = rb + j + 16 * r;
         if (MODE == 3) {
-            o[r] = caxpy(f.V[e], 0.5f * h, b[r]);
-            f.Q[e] = caxpy(f.Q[e], h / 3.0f, b[r]);
+            o[r] = caxpy(vv[r], 0.5f * h, b[r]);
+            f.Q[e] = caxpy(qv[r], h / 3.0f, b[r]);
         } else if (MODE == 4) {
             const float2 E = E_at(r);
-            o[r] = cmul(E, caxpy(f.V[e], h, b[r]));
-            f.Q[e] = cmul(E, caxpy(f.Q[e], h / 3.0f, b[r]));
+            o[r] = cmul(E, caxpy(vv[r], h, b[r]));
+            f.Q[e] = cmul(E, caxpy(qv[r], h / 3.0f, b[r]));
         } else if (MODE == 5) {
             o[r] = caxpy(f.Q[e], h / 6.0f, b[r]);
             const float a2 = cabs2(o[r]);
