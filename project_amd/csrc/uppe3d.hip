@@ -2684,7 +2684,9 @@ __global__ __launch_bounds__(1024) void colpass_kernel(const float2 *src, float2
                                                        int check_done) {
     // line pitch odd: the 16 columns q of one staging access fall on distinct bank pairs
     constexpr int EPT = NX / 64, LP = NX + NX / 16 + 1;
+    static_assert(NX <= 1024 && (NX & (NX - 1)) == 0, "one twiddle per thread");
     __shared__ float2 lines[16 * LP];
+    __shared__ float2 twl[NX];   // the stages' twiddles from LDS: no memory round trip each
     if (check_done && ctl->done) return;
     if (src_state) src = ctl->cur ? f.B1 : f.B0;
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
@@ -2692,13 +2694,15 @@ __global__ __launch_bounds__(1024) void colpass_kernel(const float2 *src, float2
     const int w = blockIdx.x / cpp, y0 = (blockIdx.x % cpp) * 16;
     const size_t base = (size_t)w * NX * ny + y0;
     const int q = tid & 15, row = tid >> 4;
+    const float2 twv = tw[tid & (NX - 1)];
+    float2 g[EPT];
 #pragma unroll
-    for (int c = 0; c < EPT; ++c) {
-        const int x = row + 64 * c;
-        lines[q * LP + pidx(x)] = src[base + (size_t)x * ny + q];
-    }
+    for (int c = 0; c < EPT; ++c) g[c] = src[base + (size_t)(row + 64 * c) * ny + q];
+    if (tid < NX) twl[tid] = twv;
+#pragma unroll
+    for (int c = 0; c < EPT; ++c) lines[q * LP + pidx(row + 64 * c)] = g[c];
     __syncthreads();
-    LineFFT<NX, EPT, 1, DIR>::run(lines + wv * LP, lane, tw);
+    LineFFT<NX, EPT, 1, DIR>::run(lines + wv * LP, lane, twl);
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < EPT; ++c) {
