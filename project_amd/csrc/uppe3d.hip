@@ -2727,22 +2727,9 @@ constexpr int PK = 65;             // transpose pitch per k (float2)
 constexpr int WREG = 16 * PK;      // float2 per wavefront
 }  // namespace r256
 
-// TL: tw0 is a copy in LDS of the 16 distinct lanes of tw256, tw0[16 k + j] = W256^(j k)
-// (one base address per lane, constant offsets).
-template <int DIR, bool TL = false>
+template <int DIR>
 __device__ __forceinline__ void twiddle256(float2 (&v)[16], int lane,
                                            const float2 *__restrict__ tw0) {
-    if constexpr (TL) {
-        const float2 *tw = tw0 + (lane & 15);
-#pragma unroll
-        for (int g = 0; g < 2; ++g) {
-#pragma unroll
-            for (int k = (g == 0 ? 1 : 8); k < (g == 0 ? 8 : 16); ++k)
-                v[k] = DIR > 0 ? cmul(v[k], tw[16 * k]) : cmulc(v[k], tw[16 * k]);
-            __builtin_amdgcn_sched_barrier(0);
-        }
-        return;
-    }
 #pragma unroll
     for (int g = 0; g < 2; ++g) {
         const float2 *tw = opaque(tw0);
@@ -2752,11 +2739,10 @@ __device__ __forceinline__ void twiddle256(float2 (&v)[16], int lane,
     }
 }
 
-template <bool TL = false>
 __device__ __forceinline__ void fwd256(float2 (&v)[16], float2 *wb, int lane,
                                        const float2 *__restrict__ tw) {
     Dft<16, 1>::run(v);
-    twiddle256<1, TL>(v, lane, tw);
+    twiddle256<1>(v, lane, tw);
 #pragma unroll
     for (int k = 0; k < 16; ++k) wb[k * r256::PK + lane] = v[k];
     wave_lds_sync();
@@ -2767,7 +2753,6 @@ __device__ __forceinline__ void fwd256(float2 (&v)[16], float2 *wb, int lane,
     Dft<16, 1>::run(v);
 }
 
-template <bool TL = false>
 __device__ __forceinline__ void inv256(float2 (&v)[16], float2 *wb, int lane,
                                        const float2 *__restrict__ tw) {
     Dft<16, -1>::run(v);   // over k2: lane (s, k1) holds u_k1[j], j < 16
@@ -2778,27 +2763,25 @@ __device__ __forceinline__ void inv256(float2 (&v)[16], float2 *wb, int lane,
 #pragma unroll
     for (int k = 0; k < 16; ++k) v[k] = wb[k1 * r256::PK + s16 + k];   // lane (s, j = k1)
     wave_lds_sync();
-    twiddle256<-1, TL>(v, lane, tw);
+    twiddle256<-1>(v, lane, tw);
     Dft<16, -1>::run(v);
 }
 
 // The row kernel with the r256 transforms: 16 rows per 256-thread workgroup, four per
 // wavefront; same modes and arithmetic.
 // a row's values y = j + 16 r (src points at the lane's first) -> forward r256 transform
-template <bool TL = false>
 __device__ __forceinline__ void row256_fwd(float2 (&v)[16], const float2 *__restrict__ src,
                                            float2 *wb, int lane, const float2 *tw) {
 #pragma unroll
     for (int r = 0; r < 16; ++r) v[r] = src[16 * r];
-    fwd256<TL>(v, wb, lane, tw);
+    fwd256(v, wb, lane, tw);
 }
 
 // inverse r256 transform -> dst[16 r] = scale v[r] (dst points at the lane's first value)
-template <bool TL = false>
 __device__ __forceinline__ void row256_inv(float2 (&v)[16], float2 *__restrict__ dst,
                                            float scale, float2 *wb, int lane,
                                            const float2 *tw) {
-    inv256<TL>(v, wb, lane, tw);
+    inv256(v, wb, lane, tw);
 #pragma unroll
     for (int r = 0; r < 16; ++r) dst[16 * r] = cscale(v[r], scale);
 }
@@ -2808,8 +2791,7 @@ __device__ __forceinline__ void row256_inv(float2 (&v)[16], float2 *__restrict__
 template <int MODE>
 __device__ __forceinline__ void row256_group(int blk, Fields3D &f, Adj3D &a, const Tabs3D &tb,
                                              const Ctl3D *ctl, float h_adj, int nx,
-                                             float2 *wregs, const float2 *tw, unsigned int &m,
-                                             double &sq) {
+                                             float2 *wregs, unsigned int &m, double &sq) {
     constexpr int NY = 256;
     constexpr bool ADJ = MODE >= 20;
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
@@ -2824,7 +2806,7 @@ __device__ __forceinline__ void row256_group(int blk, Fields3D &f, Adj3D &a, con
     const float inv = 1.0f / ((float)nx * (float)NY);
     // value r of a lane: y = j + 16 r before the forward transform and after the inverse,
     // k_y = j + 16 r in between
-    // tw: the lane twiddles in LDS (twiddle256<., true>)
+    const float2 *const tw = tb.tw256;
     // The element loops below store into fields the compiler cannot tell apart from the
     // tables, so every table or field value they need is read before them, in one batch:
     // otherwise each element would wait for a full memory round trip behind the last stores.
@@ -2838,7 +2820,7 @@ __device__ __forceinline__ void row256_group(int blk, Fields3D &f, Adj3D &a, con
     auto E_at = [&](int r) { return eprop_kp(kxx + kyv[r], 0.5f * h, vw, k2w, kw, ed, eref); };
     float2 b[16], o[16];
     const float2 *in = (MODE == 5) ? f.X : (ADJ ? a.Xb : f.Yw);
-    row256_fwd<true>(b, in + rb + j, wb, lane, tw);
+    row256_fwd(b, in + rb + j, wb, lane, tw);
     float2 vv[16], qv[16];
     if (MODE == 3 || MODE == 4) {
 #pragma unroll
@@ -2883,7 +2865,7 @@ __device__ __forceinline__ void row256_group(int blk, Fields3D &f, Adj3D &a, con
     }
     if (MODE == 2) {
         float2 av[16];
-        row256_fwd<true>(av, f.X + rb + j, wb, lane, tw);   // state, transformed along x
+        row256_fwd(av, f.X + rb + j, wb, lane, tw);   // state, transformed along x
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const size_t e = rb + j + 16 * r;
@@ -2902,11 +2884,11 @@ __device__ __forceinline__ void row256_group(int blk, Fields3D &f, Adj3D &a, con
             ab[r] = cmulc(cadd(cadd(U, a.Vb[e]), q), E);
             o[r] = cmulc(caxpy(cscale(U, 0.5f * h), h / 6.0f, q), E);
         }
-        row256_inv<true>(ab, a.Blin + rb + j, 1.0f, wb, lane, tw);
+        row256_inv(ab, a.Blin + rb + j, 1.0f, wb, lane, tw);
     }
     if (MODE == 2 || MODE == 3 || MODE == 4 || MODE == 5)
-        row256_inv<true>(o, f.X + rb + j, 1.0f / (float)NY, wb, lane, tw);
-    if (MODE == 22 || MODE == 23 || MODE == 24) row256_inv<true>(o, a.Yb + rb + j, 1.0f, wb, lane, tw);
+        row256_inv(o, f.X + rb + j, 1.0f / (float)NY, wb, lane, tw);
+    if (MODE == 22 || MODE == 23 || MODE == 24) row256_inv(o, a.Yb + rb + j, 1.0f, wb, lane, tw);
 }
 
 // The row kernel: 16 rows per 256-thread workgroup, four per wavefront.  The error-norm modes
@@ -2917,16 +2899,13 @@ __global__ __launch_bounds__(256) void row256_kernel(Fields3D f, Adj3D a, Tabs3D
                                                      float h_adj, int nx, int ngroups) {
     constexpr bool ADJ = MODE >= 20;
     __shared__ float2 wregs[4 * r256::WREG];
-    __shared__ float2 twl[256];   // twl[16 k + j] = tw256[64 k + j] = W256^(j k)
     __shared__ unsigned int red[4];
     __shared__ double reds[4];
     if (!ADJ && ctl->done) return;
-    twl[threadIdx.x] = tb.tw256[64 * (threadIdx.x >> 4) + (threadIdx.x & 15)];   // 256 threads
-    __syncthreads();
     unsigned int m = 0;
     double sq = 0.0;
     for (int blk = blockIdx.x; blk < ngroups; blk += gridDim.x)
-        row256_group<MODE>(blk, f, a, tb, ctl, h_adj, nx, wregs, twl, m, sq);
+        row256_group<MODE>(blk, f, a, tb, ctl, h_adj, nx, wregs, m, sq);
     if (MODE == 5 || MODE == 6) {
         const int tid = threadIdx.x;
         block_max_to<256>(m, MODE == 5 ? &ctl->maxA_bits : &ctl->maxD_bits, red, tid);
